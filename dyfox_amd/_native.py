@@ -1,0 +1,317 @@
+"""ctypes binding of libdyfox.so and the device array it hands out.
+
+The library is loaded from this package directory (built in-tree by
+`python -m dyfox_amd.build`).  There is no fallback: if the library or a
+device is missing, the first call raises, naming what is missing.
+
+Status codes map to the exceptions the reference raises for the same
+conditions (laue/solver.py:49-54, laue/kernels.py:365-386,
+backend.py:18-28): DYF_ENOMEM -> MemoryError, DYF_EINVAL -> ValueError,
+DYF_EKEY -> KeyError, DYF_EHIP/DYF_ENODEV -> RuntimeError.
+"""
+
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdyfox.so")
+
+DYF_OK, DYF_EINVAL, DYF_ENOMEM, DYF_EKEY, DYF_EHIP, DYF_ENODEV = \
+    0, -1, -2, -3, -4, -5
+FP64, FP32, MIXED = 0, 1, 2
+MODE_GRID, MODE_ANALYTIC, MODE_PRISTINE = 0, 1, 2
+PATH_FAST, PATH_GENERAL = 0, 1
+
+vp = C.c_void_p
+dp = C.POINTER(C.c_double)
+
+
+class VolumeDesc(C.Structure):
+    _fields_ = [("Nx", C.c_int32), ("Nz", C.c_int32),
+                ("dx", C.c_double), ("dz", C.c_double)]
+
+
+class LaueArgs(C.Structure):
+    _fields_ = [(n, C.c_double) for n in
+                ("s0r", "s0i", "shr", "shi", "sbr", "sbi", "dz", "ds", "dx",
+                 "shift_px", "sheet_hw", "fov_hy")] + \
+               [(n, C.c_int32) for n in ("Nz", "Nx", "Ny", "reserved")]
+
+
+class LineParams(C.Structure):
+    _fields_ = [(n, C.c_double) for n in
+                ("y0", "z0", "e1y", "e1z", "e2y", "e2z", "c_atan", "ge1",
+                 "ge2", "b_perp", "nu", "a2")]
+
+
+_SIGNATURES = {
+    "dyf_init": [C.c_int],
+    "dyf_device_count": [C.POINTER(C.c_int)],
+    "dyf_malloc": [C.POINTER(vp), C.c_size_t],
+    "dyf_free": [vp],
+    "dyf_memcpy_h2d": [vp, vp, C.c_size_t],
+    "dyf_memcpy_d2h": [vp, vp, C.c_size_t],
+    "dyf_memcpy_d2d": [vp, vp, C.c_size_t],
+    "dyf_memset": [vp, C.c_int, C.c_size_t],
+    "dyf_mem_info": [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)],
+    "dyf_synchronize": [],
+    "dyf_stream_sync": [vp],
+    "dyf_volterra_h": [dp, C.c_int, dp, C.c_double, C.c_double,
+                       C.POINTER(VolumeDesc), dp, C.c_int, C.c_int, vp, vp],
+    "dyf_laue_plan": [C.POINTER(LaueArgs), C.c_int, C.POINTER(C.c_int),
+                      C.POINTER(C.c_int), C.POINTER(C.c_int),
+                      C.POINTER(C.c_int)],
+    "dyf_laue_phase_bytes": [C.POINTER(LaueArgs), C.c_int, C.c_int,
+                             C.POINTER(C.c_size_t)],
+    "dyf_laue_coef_bytes": [C.POINTER(LaueArgs), C.c_int,
+                            C.POINTER(C.c_size_t)],
+    "dyf_laue_phase_from_h": [C.POINTER(LaueArgs), C.c_int, vp, C.c_int, vp,
+                              vp],
+    "dyf_laue_phase_from_segments": [C.POINTER(LaueArgs), C.c_int, dp,
+                                     C.c_int, dp, C.c_double, C.c_double, vp,
+                                     vp],
+    "dyf_laue_phase_analytic": [C.POINTER(LaueArgs), C.c_int,
+                                C.POINTER(LineParams), vp, vp],
+    "dyf_laue_march": [C.POINTER(LaueArgs), C.c_int, C.c_int, vp, C.c_int,
+                       C.POINTER(LineParams), vp, C.c_int, vp, vp, vp],
+}
+
+_lib = None
+_lock = threading.Lock()
+_initialised = False
+
+
+def declared_symbols():
+    """Names this binding expects the library to export."""
+    return sorted(_SIGNATURES) + ["dyf_last_error"]
+
+
+def load_library(path=LIB_PATH):
+    """Load libdyfox.so (no device calls).  Raises ImportError if absent."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(path):
+            raise ImportError(
+                f"{path} is not built; run `python -m dyfox_amd.build` "
+                "(hipcc, gfx950) before using the device path")
+        lib = C.CDLL(path)
+        for name, argtypes in _SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.argtypes = argtypes
+            fn.restype = C.c_int
+        lib.dyf_last_error.argtypes = []
+        lib.dyf_last_error.restype = C.c_char_p
+        _lib = lib
+        return lib
+
+
+def _raise(code, where):
+    msg = _lib.dyf_last_error().decode(errors="replace")
+    text = f"{where}: {msg}"
+    if code == DYF_ENOMEM:
+        raise MemoryError(text)
+    if code == DYF_EINVAL:
+        raise ValueError(text)
+    if code == DYF_EKEY:
+        raise KeyError(text)
+    raise RuntimeError(text)
+
+
+def call(name, *args):
+    """Call a library entry point; raise the mapped exception on failure."""
+    lib = lib_ready()
+    code = getattr(lib, name)(*args)
+    if code != DYF_OK:
+        _raise(code, name)
+
+
+def lib_ready(device=0):
+    """The loaded library with its device initialised (first call only)."""
+    global _initialised
+    lib = load_library()
+    if not _initialised:
+        with _lock:
+            if not _initialised:
+                code = lib.dyf_init(device)
+                if code != DYF_OK:
+                    _raise(code, "dyf_init")
+                _initialised = True
+    return lib
+
+
+def device_count():
+    """HIP devices visible to this process (0 without a GPU)."""
+    lib = load_library()
+    n = C.c_int(0)
+    lib.dyf_device_count(C.byref(n))
+    return n.value
+
+
+def mem_info():
+    f, t = C.c_size_t(0), C.c_size_t(0)
+    call("dyf_mem_info", C.byref(f), C.byref(t))
+    return f.value, t.value
+
+
+def synchronize():
+    call("dyf_synchronize")
+
+
+def dptr(a):
+    """ctypes double* for a C-contiguous float64 NumPy array."""
+    return a.ctypes.data_as(dp)
+
+
+class _Block:
+    """One device allocation; freed when the last view lets go."""
+
+    __slots__ = ("ptr", "nbytes")
+
+    def __init__(self, nbytes):
+        p = vp()
+        call("dyf_malloc", C.byref(p), C.c_size_t(max(int(nbytes), 1)))
+        self.ptr = p.value
+        self.nbytes = int(nbytes)
+
+    def __del__(self):
+        if self.ptr and _lib is not None:
+            _lib.dyf_free(vp(self.ptr))
+            self.ptr = None
+
+
+class DeviceArray:
+    """A C-contiguous array in device memory.
+
+    Stands where the reference returns a CuPy array: `.get()` copies to
+    the host (so `to_numpy` works unchanged), `.shape`/`.dtype` describe
+    it, and integer or slice indexing along the first axis gives a view.
+    """
+
+    def __init__(self, shape, dtype, block=None, offset=0):
+        self.shape = tuple(int(s) for s in np.atleast_1d(shape)) \
+            if not isinstance(shape, tuple) else tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * \
+            self.dtype.itemsize
+        self._block = block if block is not None else _Block(self.nbytes)
+        self._offset = int(offset)
+
+    # ---- construction ----
+    @classmethod
+    def empty(cls, shape, dtype):
+        return cls(tuple(shape), dtype)
+
+    @classmethod
+    def zeros(cls, shape, dtype):
+        a = cls(tuple(shape), dtype)
+        if a.nbytes:
+            call("dyf_memset", vp(a.ptr), 0, C.c_size_t(a.nbytes))
+        return a
+
+    @classmethod
+    def from_host(cls, host, dtype=None):
+        host = np.ascontiguousarray(host, dtype=dtype)
+        a = cls(host.shape, host.dtype)
+        if a.nbytes:
+            call("dyf_memcpy_h2d", vp(a.ptr), vp(host.ctypes.data),
+                 C.c_size_t(a.nbytes))
+        return a
+
+    # ---- properties ----
+    @property
+    def ptr(self):
+        return self._block.ptr + self._offset
+
+    @property
+    def size(self):
+        return int(np.prod(self.shape, dtype=np.int64))
+
+    @property
+    def ndim(self):
+        return len(self.shape)
+
+    def __len__(self):
+        return self.shape[0]
+
+    def get(self):
+        """Host copy (NumPy)."""
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            call("dyf_memcpy_d2h", vp(out.ctypes.data), vp(self.ptr),
+                 C.c_size_t(self.nbytes))
+        return out
+
+    def __array__(self, dtype=None, copy=None):
+        a = self.get()
+        return a if dtype is None else a.astype(dtype)
+
+    def reshape(self, *shape):
+        if len(shape) == 1 and isinstance(shape[0], (tuple, list)):
+            shape = tuple(shape[0])
+        shape = list(shape)
+        if -1 in shape:
+            j = shape.index(-1)
+            rest = int(np.prod([s for s in shape if s != -1]))
+            shape[j] = self.size // max(rest, 1)
+        if int(np.prod(shape)) != self.size:
+            raise ValueError(f"cannot reshape {self.shape} to {tuple(shape)}")
+        return DeviceArray(tuple(shape), self.dtype, self._block, self._offset)
+
+    def __getitem__(self, key):
+        if not self.shape:
+            raise IndexError("0-d device array")
+        row = int(np.prod(self.shape[1:], dtype=np.int64)) * self.dtype.itemsize
+        if isinstance(key, (int, np.integer)):
+            k = int(key)
+            if k < 0:
+                k += self.shape[0]
+            if not 0 <= k < self.shape[0]:
+                raise IndexError(key)
+            return DeviceArray(self.shape[1:], self.dtype, self._block,
+                               self._offset + k * row)
+        if isinstance(key, slice):
+            start, stop, step = key.indices(self.shape[0])
+            if step != 1:
+                raise IndexError("device views need a unit step")
+            n = max(stop - start, 0)
+            return DeviceArray((n,) + self.shape[1:], self.dtype, self._block,
+                               self._offset + start * row)
+        raise IndexError("device arrays index the first axis only")
+
+    def copy_from(self, other):
+        """Copy a device array or host array of the same size into self."""
+        if isinstance(other, DeviceArray):
+            if other.nbytes != self.nbytes:
+                raise ValueError("size mismatch")
+            call("dyf_memcpy_d2d", vp(self.ptr), vp(other.ptr),
+                 C.c_size_t(self.nbytes))
+        else:
+            host = np.ascontiguousarray(other, dtype=self.dtype)
+            if host.nbytes != self.nbytes:
+                raise ValueError("size mismatch")
+            call("dyf_memcpy_h2d", vp(self.ptr), vp(host.ctypes.data),
+                 C.c_size_t(self.nbytes))
+        return self
+
+    def astype(self, dtype):
+        """Host round trip conversion (small arrays; not on the hot path)."""
+        dtype = np.dtype(dtype)
+        if dtype == self.dtype:
+            return self
+        return DeviceArray.from_host(self.get().astype(dtype))
+
+    def __repr__(self):
+        return f"DeviceArray(shape={self.shape}, dtype={self.dtype})"
+
+
+def asarray(a, dtype=None):
+    """DeviceArray from a host array (or the same DeviceArray)."""
+    if isinstance(a, DeviceArray):
+        return a if dtype is None or np.dtype(dtype) == a.dtype \
+            else a.astype(dtype)
+    return DeviceArray.from_host(np.asarray(a), dtype=dtype)

@@ -1,0 +1,124 @@
+// Shared helpers of libdyfox: status codes, the thread-local error string,
+// HIP error checks and small complex-arithmetic helpers for device code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+
+#include "../../include/dyfox.h"
+
+namespace dyf {
+
+// Sets the thread-local message returned by dyf_last_error and returns `code`.
+int fail(int code, const char *fmt, ...);
+
+// Maps a HIP status to DYF_EHIP (or DYF_ENOMEM) with the call site named.
+int hip_fail(hipError_t e, const char *what);
+
+#define DYF_HIP(call)                                      \
+    do {                                                   \
+        hipError_t e_ = (call);                            \
+        if (e_ != hipSuccess) return dyf::hip_fail(e_, #call); \
+    } while (0)
+
+// Checks the launch status of the kernel just enqueued.
+#define DYF_LAUNCHED(name)                                 \
+    do {                                                   \
+        hipError_t e_ = hipGetLastError();                 \
+        if (e_ != hipSuccess) return dyf::hip_fail(e_, name); \
+    } while (0)
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kTwoPi = 6.28318530717958647692;
+constexpr int kWave = 64;
+
+// Complex pair in registers; plain struct so it stays in VGPRs.
+template <typename T>
+struct cx {
+    T r, i;
+};
+
+// One Volterra segment reduced to what a voxel evaluation needs (FP64).
+// Built on the host from (ra, rb, b) by prepare_segments.
+struct SegParams {
+    double cx, cy, cz;      // line point (segment midpoint)
+    double xx, xy, xz;      // unit line direction xi
+    double e1x, e1y, e1z;   // unit edge direction
+    double e2x, e2y, e2z;   // xi x e1
+    double c_atan;          // g.e1 b_perp + g.xi b_screw  (= g.b)
+    double k_xy;            // 2 pi g.e1 c1 / (2(1-nu))        * d1 d2 / r2c
+    double k_dd;            // -2 pi g.e2 c1 / (4(1-nu))       * (d1^2-d2^2)/r2c
+    double k_log;           // -2 pi g.e2 c1 (1-2nu)/(4(1-nu)) * ln r2c
+    double edge;            // 1 when b_perp > 1e-20
+    double pad_;
+};
+
+// Reduces raw (ra, rb, b) segments to SegParams; segments shorter than
+// 1e-20 m are dropped, as the reference does.  Returns the count kept.
+int prepare_segments(const double *segs, int nseg, const double *g_vec,
+                     double nu, SegParams *out);
+
+// H = 2 pi g.u at (x, y, z), segments summed in list order (FP64).
+__device__ __forceinline__ double volterra_h(const SegParams *__restrict__ s,
+                                             int nseg, double x, double y,
+                                             double z, double a2) {
+    double H = 0.0;
+    for (int n = 0; n < nseg; ++n) {
+        const SegParams &p = s[n];
+        const double X = x - p.cx, Y = y - p.cy, Z = z - p.cz;
+        const double eta = X * p.xx + Y * p.xy + Z * p.xz;
+        const double Px = X - eta * p.xx, Py = Y - eta * p.xy,
+                     Pz = Z - eta * p.xz;
+        const double d1 = Px * p.e1x + Py * p.e1y + Pz * p.e1z;
+        const double d2 = Px * p.e2x + Py * p.e2y + Pz * p.e2z;
+        double Hc = p.c_atan * atan2(d2, d1);
+        if (p.edge != 0.0) {
+            const double r2c = fmax(d1 * d1 + d2 * d2, a2);
+            const double inv = 1.0 / r2c;
+            Hc += (p.k_xy * (d1 * d2) + p.k_dd * (d1 * d1 - d2 * d2)) * inv +
+                  p.k_log * log(r2c);
+        }
+        H += Hc;
+    }
+    return H;
+}
+
+// FP32 segment accumulation (the reference's FP32 H grid: each segment's
+// FP64 value is rounded, then summed in float).
+__device__ __forceinline__ float volterra_h_f32acc(
+    const SegParams *__restrict__ s, int nseg, double x, double y, double z,
+    double a2) {
+    float H = 0.0f;
+    for (int n = 0; n < nseg; ++n) H += (float)volterra_h(s + n, 1, x, y, z, a2);
+    return H;
+}
+
+// e^{iH} rounded to float after an FP64 reduction of H into [-pi, pi].
+__device__ __forceinline__ cx<float> phase_f32(double H) {
+    const double n = rint(H * (1.0 / kTwoPi));
+    const float r = (float)fma(-n, kTwoPi, H);
+    float s, c;
+    sincosf(r, &s, &c);
+    return {c, s};
+}
+
+__device__ __forceinline__ cx<double> phase_f64(double H) {
+    double s, c;
+    sincos(H, &s, &c);
+    return {c, s};
+}
+
+// Deterministic, segment-independent grid description for kernels.
+struct RowBand {
+    int iy_lo, iy_hi;
+};
+
+// Marched-row band of a Laue grid: |(iy - Ny/2) dx| < fov_hy in FP64.
+RowBand laue_rows(const dyf_laue_args *a);
+
+}  // namespace dyf

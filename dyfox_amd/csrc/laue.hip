@@ -1,0 +1,638 @@
+// Laue (transmission) depth march on gfx950.
+//
+// The reference advances, per (s_dev, row), two amplitude rows by
+//     D0'[x] = M11 D0[x+1] + M12[x] Dg[x-1]
+//     Dg'[x] = M21[x] D0[x+1] + M22 Dg[x-1]
+// with M11 = ep (cg - i b), M22 = ep (cg + i b), M12 = ep sg sigma_hbar
+// e^{+iH}, M21 = ep sg sigma_h e^{-iH} (laue/kernels.py:90-139).
+//
+// Fast path.  M11 and M22 are the same at every column, so the march is
+// carried in the interaction picture X_k = D0_k / M11^k, Y_k = Dg_k / M22^k:
+//     X'[x] = X[x+1] + c_k e^{+iH}[x] Y[x-1]
+//     Y'[x] = Y[x-1] + b_k e^{-iH}[x] X[x+1]
+//     c_k = sg sigma_hbar/(cg - i b) e^{+2ik phi},
+//     b_k = sg sigma_h  /(cg + i b) e^{-2ik phi},  phi = atan2(b, cg)
+// and Dg = M22^(Nz-1) Y at the end.  That is 16 real FMAs per
+// pixel-step instead of 24, and every step is a small correction to the
+// state (|c_k|, |b_k| <= |sigma| ds), which is what lets the mixed mode
+// carry the state as an FP32 value plus an FP32 compensation term.
+//
+// One wavefront owns one (row, s_dev); lane l keeps columns
+// [l*C, l*C + C) of both fields in VGPRs, and the one-pixel transport is a
+// register rename inside the lane plus one cross-lane move per field per
+// step.  e^{iH} streams from the lane-interleaved table (coalesced), the
+// next step's slice prefetched while the current one is used.
+//
+// General path (any shift, any Nx): the reference's direct form, FP64, one
+// workgroup per (row, s_dev) with the rows double-buffered in LDS.
+#include "common.hpp"
+#include "phase_layout.hpp"
+
+namespace dyf {
+
+struct CoefD {
+    double cr, ci, br, bi;
+};
+struct CoefF {
+    float cr, ci, br, bi;
+};
+
+// Per-call workspace: [CoefD n_e*nsteps][CoefF n_e*nsteps][qN 2*n_e doubles]
+static size_t coef_bytes(int n_e, int nsteps) {
+    const size_t a = (size_t)n_e * nsteps * sizeof(CoefD);
+    const size_t b = (size_t)n_e * nsteps * sizeof(CoefF);
+    return a + b + (size_t)n_e * 2 * sizeof(double);
+}
+
+struct CoefPtrs {
+    CoefD *d;
+    CoefF *f;
+    double *qN;
+};
+static CoefPtrs coef_ptrs(void *ws, int n_e, int nsteps) {
+    char *p = (char *)ws;
+    CoefPtrs c;
+    c.d = (CoefD *)p;
+    c.f = (CoefF *)(p + (size_t)n_e * nsteps * sizeof(CoefD));
+    c.qN = (double *)(p + (size_t)n_e * nsteps * (sizeof(CoefD) + sizeof(CoefF)));
+    return c;
+}
+
+// One thread per (s_dev, step): the interaction-picture coefficients.
+__global__ __launch_bounds__(256) void laue_coef_kernel(
+    const double *__restrict__ s_dev, int n_e, int nsteps, double ds,
+    double s0r, double s0i, double shr, double shi, double sbr, double sbi,
+    CoefD *__restrict__ cd, CoefF *__restrict__ cf, double *__restrict__ qN) {
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx >= n_e * nsteps) return;
+    const int ie = idx / nsteps, k = idx % nsteps;
+    const double alpha = -2.0 * kPi * s_dev[ie];
+    const double gam = sqrt(shr * shr + shi * shi + alpha * alpha * 0.25);
+    double cg, sg;
+    if (gam > 1e-30) {
+        cg = cos(gam * ds);
+        sg = sin(gam * ds) / gam;
+    } else {
+        cg = 1.0;
+        sg = ds;
+    }
+    const double b = alpha * 0.5 * sg;
+    const double rho2 = cg * cg + b * b;
+    const double phi = atan2(b, cg);
+    // c0 = sg sbar (cg + i b) / rho2 ; b0 = sg sh (cg - i b) / rho2
+    const double c0r = sg * (sbr * cg - sbi * b) / rho2;
+    const double c0i = sg * (sbi * cg + sbr * b) / rho2;
+    const double b0r = sg * (shr * cg + shi * b) / rho2;
+    const double b0i = sg * (shi * cg - shr * b) / rho2;
+    double sn, cs;
+    sincos(2.0 * (double)k * phi, &sn, &cs);
+    CoefD c;
+    c.cr = c0r * cs - c0i * sn;
+    c.ci = c0r * sn + c0i * cs;
+    c.br = b0r * cs + b0i * sn;
+    c.bi = b0i * cs - b0r * sn;
+    cd[idx] = c;
+    cf[idx] = CoefF{(float)c.cr, (float)c.ci, (float)c.br, (float)c.bi};
+    if (k == 0) {
+        const double N = (double)nsteps;
+        const double mag = exp(N * ds * s0r + 0.5 * N * log(rho2));
+        const double ang = N * ds * (s0i + alpha * 0.5) + N * phi;
+        double s2, c2;
+        sincos(ang, &s2, &c2);
+        qN[2 * ie] = mag * c2;
+        qN[2 * ie + 1] = mag * s2;
+    }
+}
+
+// ---- cross-lane transport --------------------------------------------
+// value of lane+1 (0 on the last lane: nothing flows in past x = Nx)
+__device__ __forceinline__ float from_right(float v, int lane) {
+    const int w = __builtin_amdgcn_ds_bpermute((lane + 1) << 2,
+                                               __float_as_int(v));
+    return lane == kWave - 1 ? 0.0f : __int_as_float(w);
+}
+// value of lane-1 (0 on lane 0: nothing flows in before x = 0)
+__device__ __forceinline__ float from_left(float v, int lane) {
+    const int w = __builtin_amdgcn_ds_bpermute((lane - 1) << 2,
+                                               __float_as_int(v));
+    return lane == 0 ? 0.0f : __int_as_float(w);
+}
+__device__ __forceinline__ double from_right(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_ds_bpermute((lane + 1) << 2, (int)b);
+    const int hi = __builtin_amdgcn_ds_bpermute((lane + 1) << 2, (int)(b >> 32));
+    const double r = __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+    return lane == kWave - 1 ? 0.0 : r;
+}
+__device__ __forceinline__ double from_left(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_ds_bpermute((lane - 1) << 2, (int)b);
+    const int hi = __builtin_amdgcn_ds_bpermute((lane - 1) << 2, (int)(b >> 32));
+    const double r = __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+    return lane == 0 ? 0.0 : r;
+}
+
+__device__ __forceinline__ double beam_profile(int x, int Nx, double dx,
+                                               double sheet_hw) {
+    const double xp = (double)(x - Nx / 2) * dx;
+    return sheet_hw > 0.0 ? (fabs(xp) < sheet_hw ? 1.0 : 0.0)
+                          : exp(-xp * xp / (2.0 * sheet_hw * sheet_hw));
+}
+
+struct FastArgs {
+    const void *phase;   // GRID: lane-interleaved table; ANALYTIC: (nrows, nsteps)
+    const CoefD *cd;
+    const CoefF *cf;
+    const double *qN;
+    void *out;           // (n_e, Ny, Nx) complex
+    int n_e, nrows, iy_lo, nsteps, Nx, Ny, per_xcd, total;
+    double dx, sheet_hw;
+};
+
+// Block b runs on XCD b % 8; give each XCD a contiguous range of (row,
+// s_dev) pairs so that every s_dev of a row reads its e^{iH} rows through
+// the same L2.
+__device__ __forceinline__ int logical_block(int per_xcd) {
+    const int b = blockIdx.x;
+    return (b & 7) * per_xcd + (b >> 3);
+}
+
+// State element: plain (FP64 or FP32) or two-float compensated (MIXED).
+template <int PREC>
+struct StateT;
+template <>
+struct StateT<DYF_FP64> {
+    using T = double;
+    static constexpr bool comp = false;
+};
+template <>
+struct StateT<DYF_FP32> {
+    using T = float;
+    static constexpr bool comp = false;
+};
+template <>
+struct StateT<DYF_MIXED> {
+    using T = float;
+    static constexpr bool comp = true;
+};
+
+template <typename T>
+__device__ __forceinline__ T fmaT(T a, T b, T c) {
+    if constexpr (sizeof(T) == 8)
+        return __builtin_fma(a, b, c);
+    else
+        return __builtin_fmaf(a, b, c);
+}
+
+template <int C, int MODE, int PREC>
+__global__ __launch_bounds__(64) void laue_fast_kernel(FastArgs a) {
+    using S = StateT<PREC>;
+    using T = typename S::T;
+    constexpr bool COMP = S::comp;
+    const int lane = threadIdx.x;
+    const int L = logical_block(a.per_xcd);
+    if ((int)(blockIdx.x >> 3) >= a.per_xcd || L >= a.total) return;
+    const int r = L / a.n_e, ie = L - r * a.n_e;
+    const int iy = a.iy_lo + r;
+
+    // fields: hi parts (and compensations for MIXED)
+    T Xr[C], Xi[C], Yr[C], Yi[C];
+    T Xlr[COMP ? C : 1], Xli[COMP ? C : 1], Ylr[COMP ? C : 1], Yli[COMP ? C : 1];
+    T msk[MODE == DYF_MODE_GRID ? 1 : C];
+#pragma unroll
+    for (int i = 0; i < C; ++i) {
+        const int x = lane * C + i;
+        Xr[i] = x < a.Nx ? (T)beam_profile(x, a.Nx, a.dx, a.sheet_hw) : T(0);
+        Xi[i] = Yr[i] = Yi[i] = T(0);
+        if constexpr (COMP) Xlr[i] = Xli[i] = Ylr[i] = Yli[i] = T(0);
+        if constexpr (MODE != DYF_MODE_GRID) msk[i] = x < a.Nx ? T(1) : T(0);
+    }
+
+    const cx<T> *E = nullptr;
+    if constexpr (MODE == DYF_MODE_GRID)
+        E = (const cx<T> *)a.phase + (size_t)r * a.nsteps * (C * kWave) + lane;
+    const cx<T> *Erow = nullptr;
+    if constexpr (MODE == DYF_MODE_ANALYTIC)
+        Erow = (const cx<T> *)a.phase + (size_t)r * a.nsteps;
+
+    cx<T> Ec[MODE == DYF_MODE_GRID ? C : 1], En[MODE == DYF_MODE_GRID ? C : 1];
+    if constexpr (MODE == DYF_MODE_GRID) {
+#pragma unroll
+        for (int i = 0; i < C; ++i) Ec[i] = E[i * kWave];
+    }
+
+    for (int k = 0; k < a.nsteps; ++k) {
+        if constexpr (MODE == DYF_MODE_GRID) {
+            const int kn = k + 1 < a.nsteps ? k + 1 : k;
+            const cx<T> *Ek = E + (size_t)kn * (C * kWave);
+#pragma unroll
+            for (int i = 0; i < C; ++i) En[i] = Ek[i * kWave];
+        }
+        T cr, ci, br, bi;
+        if constexpr (sizeof(T) == 8) {
+            const CoefD c = a.cd[(size_t)ie * a.nsteps + k];
+            cr = c.cr, ci = c.ci, br = c.br, bi = c.bi;
+        } else {
+            const CoefF c = a.cf[(size_t)ie * a.nsteps + k];
+            cr = c.cr, ci = c.ci, br = c.br, bi = c.bi;
+        }
+        if constexpr (MODE == DYF_MODE_ANALYTIC) {
+            // fold the row-uniform e^{iH_k} into the coefficients
+            const cx<T> e = Erow[k];
+            const T ncr = cr * e.r - ci * e.i, nci = cr * e.i + ci * e.r;
+            const T nbr = br * e.r + bi * e.i, nbi = bi * e.r - br * e.i;
+            cr = ncr, ci = nci, br = nbr, bi = nbi;
+        }
+        // inflow across lane boundaries (old values)
+        const T xin_r = from_right(Xr[0], lane), xin_i = from_right(Xi[0], lane);
+        const T yin_r = from_left(Yr[C - 1], lane), yin_i = from_left(Yi[C - 1], lane);
+        T xinl_r = T(0), xinl_i = T(0), yinl_r = T(0), yinl_i = T(0);
+        if constexpr (COMP) {
+            xinl_r = from_right(Xlr[0], lane), xinl_i = from_right(Xli[0], lane);
+            yinl_r = from_left(Ylr[C - 1], lane), yinl_i = from_left(Yli[C - 1], lane);
+        }
+        T nXr[C], nXi[C], nYr[C], nYi[C];
+        T nXlr[COMP ? C : 1], nXli[COMP ? C : 1], nYlr[COMP ? C : 1], nYli[COMP ? C : 1];
+#pragma unroll
+        for (int i = 0; i < C; ++i) {
+            // sources: X from x+1, Y from x-1
+            const T xr = i + 1 < C ? Xr[i + 1] : xin_r;
+            const T xi = i + 1 < C ? Xi[i + 1] : xin_i;
+            const T yr = i > 0 ? Yr[i - 1] : yin_r;
+            const T yi = i > 0 ? Yi[i - 1] : yin_i;
+            T tr, ti, ur, ui;   // t = e^{iH} y,  u = e^{-iH} x
+            if constexpr (MODE == DYF_MODE_GRID) {
+                const T er = Ec[i].r, ei = Ec[i].i;
+                tr = fmaT(er, yr, -ei * yi);
+                ti = fmaT(er, yi, ei * yr);
+                ur = fmaT(er, xr, ei * xi);
+                ui = fmaT(er, xi, -ei * xr);
+            } else {
+                tr = msk[i] * yr, ti = msk[i] * yi;
+                ur = msk[i] * xr, ui = msk[i] * xi;
+            }
+            if constexpr (!COMP) {
+                nXr[i] = fmaT(cr, tr, fmaT(-ci, ti, xr));
+                nXi[i] = fmaT(cr, ti, fmaT(ci, tr, xi));
+                nYr[i] = fmaT(br, ur, fmaT(-bi, ui, yr));
+                nYi[i] = fmaT(br, ui, fmaT(bi, ur, yi));
+            } else {
+                const T xlr = i + 1 < C ? Xlr[i + 1] : xinl_r;
+                const T xli = i + 1 < C ? Xli[i + 1] : xinl_i;
+                const T ylr = i > 0 ? Ylr[i - 1] : yinl_r;
+                const T yli = i > 0 ? Yli[i - 1] : yinl_i;
+                // increment plus the carried compensation, then an exact
+                // (Fast2Sum) split of hi + increment
+                const T dXr = fmaT(cr, tr, fmaT(-ci, ti, xlr));
+                const T dXi = fmaT(cr, ti, fmaT(ci, tr, xli));
+                const T dYr = fmaT(br, ur, fmaT(-bi, ui, ylr));
+                const T dYi = fmaT(br, ui, fmaT(bi, ur, yli));
+                const T sXr = xr + dXr, sXi = xi + dXi;
+                const T sYr = yr + dYr, sYi = yi + dYi;
+                nXlr[i] = dXr - (sXr - xr);
+                nXli[i] = dXi - (sXi - xi);
+                nYlr[i] = dYr - (sYr - yr);
+                nYli[i] = dYi - (sYi - yi);
+                nXr[i] = sXr, nXi[i] = sXi, nYr[i] = sYr, nYi[i] = sYi;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < C; ++i) {
+            Xr[i] = nXr[i], Xi[i] = nXi[i], Yr[i] = nYr[i], Yi[i] = nYi[i];
+            if constexpr (COMP)
+                Xlr[i] = nXlr[i], Xli[i] = nXli[i], Ylr[i] = nYlr[i],
+                Yli[i] = nYli[i];
+        }
+        if constexpr (MODE == DYF_MODE_GRID) {
+#pragma unroll
+            for (int i = 0; i < C; ++i) Ec[i] = En[i];
+        }
+    }
+
+    const double qr = a.qN[2 * ie], qi = a.qN[2 * ie + 1];
+    const size_t row = ((size_t)ie * a.Ny + iy) * a.Nx;
+#pragma unroll
+    for (int i = 0; i < C; ++i) {
+        const int x = lane * C + i;
+        if (x >= a.Nx) continue;
+        double yr = (double)Yr[i], yi = (double)Yi[i];
+        if constexpr (COMP) yr += (double)Ylr[i], yi += (double)Yli[i];
+        const double o_r = qr * yr - qi * yi, o_i = qr * yi + qi * yr;
+        if constexpr (PREC == DYF_FP64)
+            ((cx<double> *)a.out)[row + x] = cx<double>{o_r, o_i};
+        else
+            ((cx<float> *)a.out)[row + x] = cx<float>{(float)o_r, (float)o_i};
+    }
+}
+
+// ---- general path: direct form, any shift, LDS rows ------------------
+struct GeneralArgs {
+    const void *H;          // GRID: (Ny, Nz, Nx) double or float
+    int H_f32;
+    dyf_line_params line;   // ANALYTIC
+    const double *s_dev;
+    void *out;
+    int out_f32;
+    int n_e, iy_lo, nrows, Nz, Nx, Ny;
+    double s0r, s0i, shr, shi, sbr, sbi, dz, ds, dx, shift, sheet_hw;
+};
+
+__device__ __forceinline__ void interp_pair(const double *re, const double *im,
+                                            double src, int Nx, double *vr,
+                                            double *vi) {
+    const int i0 = (int)floor(src);
+    const double f = src - (double)i0;
+    if (i0 >= 0 && i0 + 1 < Nx) {
+        *vr = (1.0 - f) * re[i0] + f * re[i0 + 1];
+        *vi = (1.0 - f) * im[i0] + f * im[i0 + 1];
+    } else if (i0 >= 0 && i0 < Nx) {
+        *vr = (1.0 - f) * re[i0];
+        *vi = (1.0 - f) * im[i0];
+    } else {
+        *vr = 0.0;
+        *vi = 0.0;
+    }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void laue_general_kernel(GeneralArgs a) {
+    extern __shared__ double lds[];
+    const int r = blockIdx.x / a.n_e, ie = blockIdx.x % a.n_e;
+    const int iy = a.iy_lo + r;
+    const int Nx = a.Nx, P = Nx + 1;
+    double *buf[2] = {lds, lds + 4 * P};   // [D0r, D0i, Dgr, Dgi] x 2
+    for (int x = threadIdx.x; x < Nx; x += blockDim.x) {
+        buf[0][x] = beam_profile(x, Nx, a.dx, a.sheet_hw);
+        buf[0][P + x] = buf[0][2 * P + x] = buf[0][3 * P + x] = 0.0;
+    }
+    __syncthreads();
+    const double alpha = -2.0 * kPi * a.s_dev[ie];
+    const double gam = sqrt(a.shr * a.shr + a.shi * a.shi + alpha * alpha * 0.25);
+    double cg, sg;
+    if (gam > 1e-30) {
+        cg = cos(gam * a.ds);
+        sg = sin(gam * a.ds) / gam;
+    } else {
+        cg = 1.0;
+        sg = a.ds;
+    }
+    const double ex = exp(a.s0r * a.ds), ph = (a.s0i + alpha * 0.5) * a.ds;
+    const double epr = ex * cos(ph), epi = ex * sin(ph);
+    const double b = alpha * 0.5 * sg;
+    const double M11r = epr * cg + epi * b, M11i = -epr * b + epi * cg;
+    const double M22r = epr * cg - epi * b, M22i = epr * b + epi * cg;
+    const double y_pix = (double)(iy - a.Ny / 2) * a.dx;
+    int cur = 0;
+    for (int k = 0; k < a.Nz - 1; ++k) {
+        double cHu = 1.0, sHu = 0.0;   // row-uniform phase (analytic)
+        if constexpr (MODE == DYF_MODE_ANALYTIC) {
+            const dyf_line_params &lp = a.line;
+            const double dy = y_pix - lp.y0, dzv = (double)k * a.dz - lp.z0;
+            const double d1 = dy * lp.e1y + dzv * lp.e1z;
+            const double d2 = dy * lp.e2y + dzv * lp.e2z;
+            const double r2 = d1 * d1 + d2 * d2;
+            const double r2c = r2 > lp.a2 ? r2 : lp.a2;
+            double H = lp.c_atan * atan2(d2, d1);
+            if (lp.b_perp > 0.0) {
+                const double onu = 1.0 - lp.nu, c1 = lp.b_perp / kTwoPi;
+                const double ux = c1 * d1 * d2 / (2.0 * onu * r2c);
+                const double uy = -c1 * ((1.0 - 2.0 * lp.nu) / (4.0 * onu) * log(r2c) +
+                                         (d1 * d1 - d2 * d2) / (4.0 * onu * r2c));
+                H += kTwoPi * (lp.ge1 * ux + lp.ge2 * uy);
+            }
+            sincos(H, &sHu, &cHu);
+        }
+        const double *s = buf[cur];
+        double *d = buf[1 - cur];
+        for (int x = threadIdx.x; x < Nx; x += blockDim.x) {
+            double cH = cHu, sH = sHu;
+            if constexpr (MODE == DYF_MODE_GRID) {
+                const size_t at = ((size_t)iy * a.Nz + k) * Nx + x;
+                const double h = a.H_f32 ? (double)((const float *)a.H)[at]
+                                         : ((const double *)a.H)[at];
+                sincos(h, &sH, &cH);
+            }
+            const double P12r = (a.sbr * cH - a.sbi * sH) * sg;
+            const double P12i = (a.sbi * cH + a.sbr * sH) * sg;
+            const double P21r = (a.shr * cH + a.shi * sH) * sg;
+            const double P21i = (a.shi * cH - a.shr * sH) * sg;
+            const double M12r = epr * P12r - epi * P12i, M12i = epr * P12i + epi * P12r;
+            const double M21r = epr * P21r - epi * P21i, M21i = epr * P21i + epi * P21r;
+            double d0r, d0i, dgr, dgi;
+            interp_pair(s, s + P, (double)x + a.shift, Nx, &d0r, &d0i);
+            interp_pair(s + 2 * P, s + 3 * P, (double)x - a.shift, Nx, &dgr, &dgi);
+            d[x] = M11r * d0r - M11i * d0i + M12r * dgr - M12i * dgi;
+            d[P + x] = M11r * d0i + M11i * d0r + M12r * dgi + M12i * dgr;
+            d[2 * P + x] = M21r * d0r - M21i * d0i + M22r * dgr - M22i * dgi;
+            d[3 * P + x] = M21r * d0i + M21i * d0r + M22r * dgi + M22i * dgr;
+        }
+        __syncthreads();
+        cur = 1 - cur;
+    }
+    const double *s = buf[cur];
+    const size_t row = ((size_t)ie * a.Ny + iy) * Nx;
+    for (int x = threadIdx.x; x < Nx; x += blockDim.x) {
+        if (a.out_f32)
+            ((cx<float> *)a.out)[row + x] =
+                cx<float>{(float)s[2 * P + x], (float)s[3 * P + x]};
+        else
+            ((cx<double> *)a.out)[row + x] = cx<double>{s[2 * P + x], s[3 * P + x]};
+    }
+}
+
+// Zeroes the rows outside the field of view (a fill kernel rather than a
+// memset, so the whole solve can be captured in a graph).
+__global__ __launch_bounds__(256) void zero_rows_kernel(void *out, int elem,
+                                                        int n_e, int Ny, int Nx,
+                                                        int iy_lo, int iy_hi) {
+    const size_t per = (size_t)(Ny - (iy_hi - iy_lo)) * Nx;
+    const size_t n = per * n_e;
+    for (size_t t = (size_t)blockIdx.x * 256 + threadIdx.x; t < n;
+         t += (size_t)gridDim.x * 256) {
+        const int ie = (int)(t / per);
+        const size_t rem = t % per;
+        int iy = (int)(rem / Nx);
+        const int x = (int)(rem % Nx);
+        if (iy >= iy_lo) iy += iy_hi - iy_lo;
+        const size_t at = ((size_t)ie * Ny + iy) * Nx + x;
+        if (elem == 16)
+            ((cx<double> *)out)[at] = cx<double>{0.0, 0.0};
+        else
+            ((cx<float> *)out)[at] = cx<float>{0.0f, 0.0f};
+    }
+}
+
+// ---- fast-kernel dispatch table ---------------------------------------
+using FastFn = void (*)(FastArgs);
+
+template <int MODE, int PREC, int... Cs>
+struct FastTable {
+    static FastFn get(int C) {
+        FastFn f = nullptr;
+        ((C == Cs ? (f = laue_fast_kernel<Cs, MODE, PREC>, 0) : 0), ...);
+        return f;
+    }
+};
+
+template <int MODE, int PREC>
+static FastFn fast_fn(int C) {
+    return FastTable<MODE, PREC, 2, 4, 6, 8, 10, 12, 13, 14, 16, 17, 18>::get(C);
+}
+
+static FastFn pick_fast(int mode, int prec, int C) {
+#define DYF_PICK(M, P) \
+    if (mode == M && prec == P) return fast_fn<M, P>(C);
+    DYF_PICK(DYF_MODE_GRID, DYF_FP64)
+    DYF_PICK(DYF_MODE_GRID, DYF_FP32)
+    DYF_PICK(DYF_MODE_GRID, DYF_MIXED)
+    DYF_PICK(DYF_MODE_ANALYTIC, DYF_FP64)
+    DYF_PICK(DYF_MODE_ANALYTIC, DYF_FP32)
+    DYF_PICK(DYF_MODE_ANALYTIC, DYF_MIXED)
+    DYF_PICK(DYF_MODE_PRISTINE, DYF_FP64)
+    DYF_PICK(DYF_MODE_PRISTINE, DYF_FP32)
+    DYF_PICK(DYF_MODE_PRISTINE, DYF_MIXED)
+#undef DYF_PICK
+    return nullptr;
+}
+
+}  // namespace dyf
+
+using namespace dyf;
+
+extern "C" int dyf_laue_plan(const dyf_laue_args *a, int prec, int *path,
+                             int *cols, int *iy_lo, int *iy_hi) {
+    PhaseLayout L;
+    int rc = phase_layout(a, DYF_MODE_GRID, prec, &L);
+    if (rc) return rc;
+    if (path) *path = L.path;
+    if (cols) *cols = L.path == DYF_PATH_FAST ? L.C : 0;
+    if (iy_lo) *iy_lo = L.iy_lo;
+    if (iy_hi) *iy_hi = L.iy_hi;
+    return DYF_OK;
+}
+
+extern "C" int dyf_laue_phase_bytes(const dyf_laue_args *a, int mode, int prec,
+                                    size_t *bytes) {
+    PhaseLayout L;
+    int rc = phase_layout(a, mode, prec, &L);
+    if (rc) return rc;
+    if (!bytes) return fail(DYF_EINVAL, "bytes is NULL");
+    *bytes = L.bytes;
+    return DYF_OK;
+}
+
+extern "C" int dyf_laue_coef_bytes(const dyf_laue_args *a, int n_e,
+                                   size_t *bytes) {
+    int rc = check_laue_args(a);
+    if (rc) return rc;
+    if (n_e < 0 || !bytes) return fail(DYF_EINVAL, "bad n_e or bytes");
+    *bytes = coef_bytes(n_e, a->Nz - 1);
+    return DYF_OK;
+}
+
+extern "C" int dyf_laue_march(const dyf_laue_args *a, int mode, int prec,
+                              const void *src, int H_prec,
+                              const dyf_line_params *line,
+                              const double *s_dev, int n_e, void *coef,
+                              void *out, void *stream_) {
+    hipStream_t stream = (hipStream_t)stream_;
+    PhaseLayout L;
+    int rc = phase_layout(a, mode, prec, &L);
+    if (rc) return rc;
+    if (n_e < 0) return fail(DYF_EINVAL, "n_e < 0");
+    if (n_e == 0) return DYF_OK;
+    if (!out || !s_dev) return fail(DYF_EINVAL, "NULL output or s_dev");
+    const int elem = prec == DYF_FP64 ? 16 : 8;
+    const int nrows = L.nrows;
+    if (nrows < a->Ny) {
+        const size_t n = (size_t)(a->Ny - nrows) * a->Nx * n_e;
+        const int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
+        zero_rows_kernel<<<blocks, 256, 0, stream>>>(out, elem, n_e, a->Ny,
+                                                      a->Nx, L.iy_lo, L.iy_hi);
+        DYF_LAUNCHED("zero_rows_kernel");
+    }
+    if (nrows == 0) return DYF_OK;
+    if (L.path == DYF_PATH_FAST) {
+        if (mode != DYF_MODE_PRISTINE && !src)
+            return fail(DYF_EINVAL, "phase table is NULL");
+        if (!coef) return fail(DYF_EINVAL, "coefficient workspace is NULL");
+        const CoefPtrs cp = coef_ptrs(coef, n_e, L.nsteps);
+        const int ncoef = n_e * L.nsteps;
+        laue_coef_kernel<<<(ncoef + 255) / 256, 256, 0, stream>>>(
+            s_dev, n_e, L.nsteps, a->ds, a->s0r, a->s0i, a->shr, a->shi,
+            a->sbr, a->sbi, cp.d, cp.f, cp.qN);
+        DYF_LAUNCHED("laue_coef_kernel");
+        FastArgs fa;
+        fa.phase = src;
+        fa.cd = cp.d;
+        fa.cf = cp.f;
+        fa.qN = cp.qN;
+        fa.out = out;
+        fa.n_e = n_e;
+        fa.nrows = nrows;
+        fa.iy_lo = L.iy_lo;
+        fa.nsteps = L.nsteps;
+        fa.Nx = a->Nx;
+        fa.Ny = a->Ny;
+        fa.total = nrows * n_e;
+        fa.per_xcd = (fa.total + 7) / 8;
+        fa.dx = a->dx;
+        fa.sheet_hw = a->sheet_hw;
+        FastFn fn = pick_fast(mode, prec, L.C);
+        if (!fn) return fail(DYF_EKEY, "no fast kernel for C=%d", L.C);
+        hipLaunchKernelGGL(fn, dim3(8 * fa.per_xcd), dim3(kWave), 0, stream, fa);
+        DYF_LAUNCHED("laue_fast_kernel");
+        return DYF_OK;
+    }
+    // general path
+    if (mode == DYF_MODE_GRID && !src) return fail(DYF_EINVAL, "H grid is NULL");
+    if (mode == DYF_MODE_GRID && H_prec != DYF_FP64 && H_prec != DYF_FP32)
+        return fail(DYF_EKEY, "H precision must be DYF_FP64 or DYF_FP32");
+    if (mode == DYF_MODE_ANALYTIC && !line)
+        return fail(DYF_EINVAL, "line parameters are NULL");
+    const size_t lds = (size_t)2 * 4 * (a->Nx + 1) * sizeof(double);
+    if (lds > 160 * 1024)
+        return fail(DYF_ENOMEM,
+                    "the general march needs %zu KB of LDS per workgroup but "
+                    "the device allows 160 KB; reduce the x-grid",
+                    lds / 1024);
+    GeneralArgs ga;
+    ga.H = src;
+    ga.H_f32 = H_prec == DYF_FP32;
+    if (line) ga.line = *line;
+    ga.s_dev = s_dev;
+    ga.out = out;
+    ga.out_f32 = prec != DYF_FP64;
+    ga.n_e = n_e;
+    ga.iy_lo = L.iy_lo;
+    ga.nrows = nrows;
+    ga.Nz = a->Nz;
+    ga.Nx = a->Nx;
+    ga.Ny = a->Ny;
+    ga.s0r = a->s0r, ga.s0i = a->s0i, ga.shr = a->shr, ga.shi = a->shi;
+    ga.sbr = a->sbr, ga.sbi = a->sbi;
+    ga.dz = a->dz, ga.ds = a->ds, ga.dx = a->dx, ga.shift = a->shift_px;
+    ga.sheet_hw = a->sheet_hw;
+    dim3 grid(nrows * n_e);
+    if (mode == DYF_MODE_GRID) {
+        DYF_HIP(hipFuncSetAttribute((const void *)laue_general_kernel<DYF_MODE_GRID>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds));
+        hipLaunchKernelGGL(laue_general_kernel<DYF_MODE_GRID>, grid, dim3(256),
+                           lds, stream, ga);
+    } else if (mode == DYF_MODE_ANALYTIC) {
+        DYF_HIP(hipFuncSetAttribute(
+            (const void *)laue_general_kernel<DYF_MODE_ANALYTIC>,
+            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(laue_general_kernel<DYF_MODE_ANALYTIC>, grid,
+                           dim3(256), lds, stream, ga);
+    } else {
+        DYF_HIP(hipFuncSetAttribute(
+            (const void *)laue_general_kernel<DYF_MODE_PRISTINE>,
+            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(laue_general_kernel<DYF_MODE_PRISTINE>, grid,
+                           dim3(256), lds, stream, ga);
+    }
+    DYF_LAUNCHED("laue_general_kernel");
+    return DYF_OK;
+}

@@ -1,0 +1,77 @@
+// How a Laue grid maps onto the march kernels, shared by the phase-table
+// producers and the march itself so both agree on the table layout.
+//
+// Fast path (exact one-pixel shift): one wavefront per (row, s_dev); lane
+// l holds columns [l*C, l*C + C) in registers.  The e^{iH} table of a row
+// and step is stored lane-interleaved, element (i, lane) at i*64 + lane,
+// so each of the C loads per step is one contiguous, coalesced 64-wide
+// access.  Columns x >= Nx hold e^{iH} = 0, which keeps the padding
+// columns from coupling back into the grid.
+#pragma once
+
+#include "common.hpp"
+
+namespace dyf {
+
+// Column counts per lane that have compiled fast kernels, per precision
+// family (register budget: FP64 state + phase prefetch ~12 VGPRs/column,
+// two-float state ~10 VGPRs/column).
+constexpr int kFastCols[] = {2, 4, 6, 8, 10, 12, 13, 14, 16, 17, 18};
+constexpr int kNumFastCols = sizeof(kFastCols) / sizeof(kFastCols[0]);
+
+struct PhaseLayout {
+    int path;       // DYF_PATH_FAST or DYF_PATH_GENERAL
+    int C;          // columns per lane (fast path)
+    int iy_lo, iy_hi, nrows;
+    int nsteps;     // Nz - 1
+    size_t bytes;   // phase buffer size for (mode, prec)
+};
+
+inline int fast_cols_for(int Nx) {
+    const int need = (Nx + kWave - 1) / kWave;
+    for (int j = 0; j < kNumFastCols; ++j)
+        if (kFastCols[j] >= need) return kFastCols[j];
+    return 0;
+}
+
+inline int check_laue_args(const dyf_laue_args *a) {
+    if (!a) return fail(DYF_EINVAL, "laue args are NULL");
+    if (a->Nz < 2 || a->Nx < 1 || a->Ny < 1)
+        return fail(DYF_EINVAL, "bad grid Nz=%d Nx=%d Ny=%d", a->Nz, a->Nx,
+                    a->Ny);
+    if (!(a->ds > 0.0) || !(a->dx > 0.0))
+        return fail(DYF_EINVAL, "ds and dx must be positive");
+    return DYF_OK;
+}
+
+inline int phase_layout(const dyf_laue_args *a, int mode, int prec,
+                        PhaseLayout *L) {
+    int rc = check_laue_args(a);
+    if (rc) return rc;
+    if (prec != DYF_FP64 && prec != DYF_FP32 && prec != DYF_MIXED)
+        return fail(DYF_EKEY, "unknown precision %d", prec);
+    if (mode != DYF_MODE_GRID && mode != DYF_MODE_ANALYTIC &&
+        mode != DYF_MODE_PRISTINE)
+        return fail(DYF_EKEY, "unknown march mode %d", mode);
+    const RowBand band = laue_rows(a);
+    L->iy_lo = band.iy_lo;
+    L->iy_hi = band.iy_hi;
+    L->nrows = band.iy_hi - band.iy_lo;
+    L->nsteps = a->Nz - 1;
+    L->C = fast_cols_for(a->Nx);
+    // The interaction-picture march divides by the diagonal propagator;
+    // keep it well conditioned: |sigma_h| ds small (always true on exact
+    // grids at sub-micron pitch).
+    const double sh = std::sqrt(a->shr * a->shr + a->shi * a->shi);
+    const bool fast = a->shift_px == 1.0 && L->C > 0 && sh * a->ds < 0.25;
+    L->path = fast ? DYF_PATH_FAST : DYF_PATH_GENERAL;
+    const size_t elem = prec == DYF_FP64 ? 16 : 8;
+    L->bytes = 0;
+    if (fast && mode == DYF_MODE_GRID)
+        L->bytes = (size_t)L->nrows * L->nsteps * L->C * kWave * elem;
+    else if (fast && mode == DYF_MODE_ANALYTIC)
+        L->bytes = (size_t)L->nrows * L->nsteps * elem;
+    return DYF_OK;
+}
+
+}  // namespace dyf

@@ -1,0 +1,258 @@
+// Volterra displacement phase H = 2 pi g.u of straight dislocations
+// (isotropic elasticity, reference dislocations.py:113-211), evaluated in
+// FP64 on gfx950.  Three producers:
+//   * the H grid (ny, Nz, Nx) of compute_H_grid / H_plane;
+//   * the lane-interleaved e^{iH} table the fast Laue march streams;
+//   * e^{iH(y, z_k)} per row and step for a line along x_lab.
+// Segments live in constant memory (uniform across every wave, so every
+// read is a scalar load); calls on one device are serialised by stream
+// order, which is what makes one symbol per library enough.
+#include "common.hpp"
+#include "phase_layout.hpp"
+
+namespace dyf {
+
+constexpr int kMaxSeg = 224;
+constexpr int kMaxRows = 2048;
+__constant__ SegParams c_segs[kMaxSeg];
+__constant__ double c_yvals[kMaxRows];
+
+static int upload_segments(const double *segs, int nseg, const double *g,
+                           double nu, int *kept, hipStream_t stream) {
+    if (nseg < 0 || (nseg > 0 && !segs))
+        return fail(DYF_EINVAL, "bad segment list (n=%d)", nseg);
+    if (nseg > kMaxSeg)
+        return fail(DYF_EINVAL, "%d segments exceed the limit of %d", nseg,
+                    kMaxSeg);
+    if (!g) return fail(DYF_EINVAL, "g_vec is NULL");
+    SegParams host[kMaxSeg];
+    *kept = prepare_segments(segs, nseg, g, nu, host);
+    // Synchronous: the host array is gone once this call returns, and a
+    // later call must not overwrite the symbol before this one's kernels
+    // ran (the copy is ordered after all prior work on the device).
+    if (*kept > 0)
+        DYF_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_segs), host,
+                                  sizeof(SegParams) * (*kept), 0,
+                                  hipMemcpyHostToDevice));
+    (void)stream;
+    return DYF_OK;
+}
+
+// ---- H grid: one thread per voxel, x fastest (coalesced stores) -------
+template <typename OutT>
+__global__ __launch_bounds__(256) void volterra_grid_kernel(
+    OutT *__restrict__ H, int nseg,
+    int Nx, int Nz, double dx, double dz, double a2) {
+    const int ix = blockIdx.x * 256 + threadIdx.x;
+    const int k = blockIdx.y;
+    const int j = blockIdx.z;
+    if (ix >= Nx) return;
+    const double x = ((double)ix - (double)Nx / 2.0) * dx;
+    const double z = (double)k * dz;
+    const double y = c_yvals[j];
+    OutT h;
+    if constexpr (sizeof(OutT) == 8)
+        h = volterra_h(c_segs, nseg, x, y, z, a2);
+    else
+        h = volterra_h_f32acc(c_segs, nseg, x, y, z, a2);
+    H[((size_t)j * Nz + k) * Nx + ix] = h;
+}
+
+// ---- fused e^{iH} table for the fast march ---------------------------
+// Thread t of a (row, step) covers x = lane*C + i with i = t / 64,
+// lane = t % 64, and writes element t of that (row, step) block.
+template <typename T>
+__global__ __launch_bounds__(256) void phase_from_segments_kernel(
+    cx<T> *__restrict__ E, int nseg, int C, int Nx, int Ny, int nsteps,
+    int iy_lo, double dx, double dz, double a2) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int k = blockIdx.y;
+    const int r = blockIdx.z;
+    if (t >= C * kWave) return;
+    const int i = t / kWave, lane = t % kWave;
+    const int x = lane * C + i;
+    cx<T> e{T(0), T(0)};
+    if (x < Nx) {
+        const double xp = ((double)x - (double)Nx / 2.0) * dx;
+        const double yp = (double)(iy_lo + r - Ny / 2) * dx;
+        const double H = volterra_h(c_segs, nseg, xp, yp, (double)k * dz, a2);
+        if constexpr (sizeof(T) == 8)
+            e = phase_f64(H);
+        else
+            e = phase_f32(H);
+    }
+    E[((size_t)r * nsteps + k) * (size_t)(C * kWave) + t] = e;
+}
+
+// ---- e^{iH(y, z_k)} for one line along x_lab (the analytic kernel's
+// in-loop phase, laue/kernels.py:216-229), one value per (row, step) ---
+template <typename T>
+__global__ __launch_bounds__(256) void phase_analytic_kernel(
+    cx<T> *__restrict__ E, dyf_line_params lp, int Ny, int nsteps,
+    int nrows, int iy_lo, double dx, double dz) {
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx >= nrows * nsteps) return;
+    const int r = idx / nsteps, k = idx % nsteps;
+    const double dy = (double)(iy_lo + r - Ny / 2) * dx - lp.y0;
+    const double dzv = (double)k * dz - lp.z0;
+    const double d1 = dy * lp.e1y + dzv * lp.e1z;
+    const double d2 = dy * lp.e2y + dzv * lp.e2z;
+    const double r2 = d1 * d1 + d2 * d2;
+    const double r2c = r2 > lp.a2 ? r2 : lp.a2;
+    double H = lp.c_atan * atan2(d2, d1);
+    if (lp.b_perp > 0.0) {
+        const double onu = 1.0 - lp.nu, c1 = lp.b_perp / kTwoPi;
+        const double ux = c1 * d1 * d2 / (2.0 * onu * r2c);
+        const double uy = -c1 * ((1.0 - 2.0 * lp.nu) / (4.0 * onu) * log(r2c) +
+                                 (d1 * d1 - d2 * d2) / (4.0 * onu * r2c));
+        H += kTwoPi * (lp.ge1 * ux + lp.ge2 * uy);
+    }
+    if constexpr (sizeof(T) == 8)
+        E[idx] = phase_f64(H);
+    else
+        E[idx] = phase_f32(H);
+}
+
+// ---- e^{iH} table from a stored H grid (the H_gpu= path) --------------
+template <typename T, typename HT>
+__global__ __launch_bounds__(256) void phase_from_h_kernel(
+    cx<T> *__restrict__ E, const HT *__restrict__ H, int C, int Nx, int Nz,
+    int nsteps, int iy_lo) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int k = blockIdx.y;
+    const int r = blockIdx.z;
+    if (t >= C * kWave) return;
+    const int i = t / kWave, lane = t % kWave;
+    const int x = lane * C + i;
+    cx<T> e{T(0), T(0)};
+    if (x < Nx) {
+        const double h = (double)H[((size_t)(iy_lo + r) * Nz + k) * Nx + x];
+        if constexpr (sizeof(T) == 8)
+            e = phase_f64(h);
+        else
+            e = phase_f32(h);
+    }
+    E[((size_t)r * nsteps + k) * (size_t)(C * kWave) + t] = e;
+}
+
+}  // namespace dyf
+
+using namespace dyf;
+
+extern "C" int dyf_volterra_h(const double *segs, int nseg, const double *g,
+                              double nu, double a_core, const dyf_volume *vol,
+                              const double *y_values, int ny, int out_prec,
+                              void *H, void *stream_) {
+    hipStream_t stream = (hipStream_t)stream_;
+    if (!vol || vol->Nx <= 0 || vol->Nz <= 0)
+        return fail(DYF_EINVAL, "bad volume");
+    if (ny < 0 || (ny > 0 && (!y_values || !H)))
+        return fail(DYF_EINVAL, "bad rows (ny=%d)", ny);
+    if (out_prec != DYF_FP64 && out_prec != DYF_FP32)
+        return fail(DYF_EKEY, "H precision must be DYF_FP64 or DYF_FP32");
+    if (ny == 0) return DYF_OK;
+    int kept = 0;
+    int rc = upload_segments(segs, nseg, g, nu, &kept, stream);
+    if (rc) return rc;
+    if (vol->Nz > 65535)
+        return fail(DYF_EINVAL, "grid too large for one launch");
+    const double a2 = a_core * a_core;
+    // rows go through constant memory in blocks of kMaxRows
+    for (int j0 = 0; j0 < ny; j0 += kMaxRows) {
+        const int nb = std::min(kMaxRows, ny - j0);
+        DYF_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_yvals), y_values + j0,
+                                  sizeof(double) * nb, 0,
+                                  hipMemcpyHostToDevice));
+        dim3 grid((vol->Nx + 255) / 256, vol->Nz, nb);
+        const size_t off = (size_t)j0 * vol->Nz * vol->Nx;
+        if (out_prec == DYF_FP64)
+            volterra_grid_kernel<double><<<grid, 256, 0, stream>>>(
+                (double *)H + off, kept, vol->Nx, vol->Nz, vol->dx, vol->dz, a2);
+        else
+            volterra_grid_kernel<float><<<grid, 256, 0, stream>>>(
+                (float *)H + off, kept, vol->Nx, vol->Nz, vol->dx, vol->dz, a2);
+        DYF_LAUNCHED("volterra_grid_kernel");
+    }
+    return DYF_OK;
+}
+
+extern "C" int dyf_laue_phase_from_segments(const dyf_laue_args *a, int prec,
+                                            const double *segs, int nseg,
+                                            const double *g, double nu,
+                                            double a_core, void *phase,
+                                            void *stream_) {
+    hipStream_t stream = (hipStream_t)stream_;
+    PhaseLayout L;
+    int rc = phase_layout(a, DYF_MODE_GRID, prec, &L);
+    if (rc) return rc;
+    if (L.path != DYF_PATH_FAST)
+        return fail(DYF_EINVAL, "grid takes the general path; build an H grid");
+    if (L.nrows == 0) return DYF_OK;
+    if (!phase) return fail(DYF_EINVAL, "phase buffer is NULL");
+    int kept = 0;
+    rc = upload_segments(segs, nseg, g, nu, &kept, stream);
+    if (rc) return rc;
+    dim3 grid((L.C * kWave + 255) / 256, L.nsteps, L.nrows);
+    const double a2 = a_core * a_core;
+    if (prec == DYF_FP64)
+        phase_from_segments_kernel<double><<<grid, 256, 0, stream>>>(
+            (cx<double> *)phase, kept, L.C, a->Nx, a->Ny, L.nsteps, L.iy_lo,
+            a->dx, a->dz, a2);
+    else
+        phase_from_segments_kernel<float><<<grid, 256, 0, stream>>>(
+            (cx<float> *)phase, kept, L.C, a->Nx, a->Ny, L.nsteps, L.iy_lo,
+            a->dx, a->dz, a2);
+    DYF_LAUNCHED("phase_from_segments_kernel");
+    return DYF_OK;
+}
+
+extern "C" int dyf_laue_phase_analytic(const dyf_laue_args *a, int prec,
+                                       const dyf_line_params *line,
+                                       void *phase, void *stream_) {
+    hipStream_t stream = (hipStream_t)stream_;
+    PhaseLayout L;
+    int rc = phase_layout(a, DYF_MODE_ANALYTIC, prec, &L);
+    if (rc) return rc;
+    if (!line) return fail(DYF_EINVAL, "line parameters are NULL");
+    if (L.path != DYF_PATH_FAST)
+        return fail(DYF_EINVAL, "grid takes the general path");
+    const int n = L.nrows * L.nsteps;
+    if (n == 0) return DYF_OK;
+    if (!phase) return fail(DYF_EINVAL, "phase buffer is NULL");
+    if (prec == DYF_FP64)
+        phase_analytic_kernel<double><<<(n + 255) / 256, 256, 0, stream>>>(
+            (cx<double> *)phase, *line, a->Ny, L.nsteps, L.nrows, L.iy_lo,
+            a->dx, a->dz);
+    else
+        phase_analytic_kernel<float><<<(n + 255) / 256, 256, 0, stream>>>(
+            (cx<float> *)phase, *line, a->Ny, L.nsteps, L.nrows, L.iy_lo,
+            a->dx, a->dz);
+    DYF_LAUNCHED("phase_analytic_kernel");
+    return DYF_OK;
+}
+
+extern "C" int dyf_laue_phase_from_h(const dyf_laue_args *a, int prec,
+                                     const void *H, int H_prec, void *phase,
+                                     void *stream_) {
+    hipStream_t stream = (hipStream_t)stream_;
+    PhaseLayout L;
+    int rc = phase_layout(a, DYF_MODE_GRID, prec, &L);
+    if (rc) return rc;
+    if (L.path != DYF_PATH_FAST)
+        return fail(DYF_EINVAL, "grid takes the general path; march H directly");
+    if (H_prec != DYF_FP64 && H_prec != DYF_FP32)
+        return fail(DYF_EKEY, "H precision must be DYF_FP64 or DYF_FP32");
+    if (L.nrows == 0) return DYF_OK;
+    if (!H || !phase) return fail(DYF_EINVAL, "NULL H or phase buffer");
+    dim3 grid((L.C * kWave + 255) / 256, L.nsteps, L.nrows);
+#define DYF_PFH(T, HT)                                                      \
+    phase_from_h_kernel<T, HT><<<grid, 256, 0, stream>>>(                  \
+        (cx<T> *)phase, (const HT *)H, L.C, a->Nx, a->Nz, L.nsteps, L.iy_lo)
+    if (prec == DYF_FP64 && H_prec == DYF_FP64) DYF_PFH(double, double);
+    else if (prec == DYF_FP64) DYF_PFH(double, float);
+    else if (H_prec == DYF_FP64) DYF_PFH(float, double);
+    else DYF_PFH(float, float);
+#undef DYF_PFH
+    DYF_LAUNCHED("phase_from_h_kernel");
+    return DYF_OK;
+}

@@ -1,0 +1,136 @@
+/*
+ * dyfox.h -- C ABI of libdyfox.so, the MI355X (gfx950) Takagi-Taupin engine.
+ *
+ * Every entry point returns an int status (DYF_OK = 0, negative on error);
+ * dyf_last_error() gives a thread-local message for the last failure.
+ * All pointers named *_dev are device pointers obtained from dyf_malloc
+ * (or any HIP allocation); every other pointer is host memory.  `stream`
+ * is a hipStream_t (NULL = the legacy default stream).  Launches are
+ * asynchronous on `stream`; call dyf_synchronize() or dyf_stream_sync()
+ * before reading results on the host.
+ *
+ * The reference (DYFOX) has no compiled boundary: its kernels are CUDA C
+ * strings compiled by CuPy/NVRTC and launched with positional argument
+ * tuples.  Each declaration below cites the reference call it replaces.
+ */
+#ifndef DYFOX_H
+#define DYFOX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------- */
+enum {
+    DYF_OK = 0,
+    DYF_EINVAL = -1,   /* bad argument                 -> ValueError  */
+    DYF_ENOMEM = -2,   /* device memory / LDS exceeded -> MemoryError */
+    DYF_EKEY = -3,     /* unknown mode or precision    -> KeyError    */
+    DYF_EHIP = -4,     /* HIP runtime error            -> RuntimeError */
+    DYF_ENODEV = -5    /* no usable device             -> RuntimeError */
+};
+
+/* ---- precisions and march modes ------------------------------------- */
+enum {
+    DYF_FP64 = 0,   /* complex128 amplitudes, FP64 phase                   */
+    DYF_FP32 = 1,   /* complex64 amplitudes, FP32 state, FP64-reduced phase */
+    DYF_MIXED = 2   /* complex64 out; FP32 two-float compensated state      */
+};
+enum {
+    DYF_MODE_GRID = 0,      /* per-voxel H, any segments   (tt3d_fullfield) */
+    DYF_MODE_ANALYTIC = 1,  /* one line along x_lab, H(y,z) (tt3d_analytic)  */
+    DYF_MODE_PRISTINE = 2   /* H = 0                        (tt3d_pristine)  */
+};
+enum {
+    DYF_PATH_FAST = 0,      /* exact one-pixel shift, register-resident rows */
+    DYF_PATH_GENERAL = 1    /* any shift (linear interpolation), LDS rows    */
+};
+
+/* ---- runtime (replaces CuPy allocation/copies, backend.py:18-47) ----- */
+int dyf_init(int device);
+const char *dyf_last_error(void);
+int dyf_device_count(int *count);
+int dyf_malloc(void **ptr_dev, size_t bytes);
+int dyf_free(void *ptr_dev);
+int dyf_memcpy_h2d(void *dst_dev, const void *src, size_t bytes);
+int dyf_memcpy_d2h(void *dst, const void *src_dev, size_t bytes);
+int dyf_memcpy_d2d(void *dst_dev, const void *src_dev, size_t bytes);
+int dyf_memset(void *ptr_dev, int value, size_t bytes);
+int dyf_mem_info(size_t *free_bytes, size_t *total_bytes);
+int dyf_synchronize(void);
+int dyf_stream_sync(void *stream);
+
+/* ---- Volterra displacement phase ------------------------------------ */
+/* One straight dislocation segment, lab frame, metres: ra[3], rb[3], b[3]
+ * (9 doubles per segment, the reference's (ra, rb, b) tuple).            */
+typedef struct {
+    int32_t Nx, Nz;   /* x columns, depth planes                  */
+    double dx, dz;    /* x pitch, depth step (m)                  */
+} dyf_volume;
+
+/* H = 2 pi g.u on rows y_values[0..ny) of a (ny, Nz, Nx) block, x =
+ * (ix - Nx/2) dx, z = k dz.  out_prec DYF_FP64 writes double and sums
+ * segments in FP64; DYF_FP32 writes float and accumulates each segment's
+ * FP64 value in FP32.  Replaces compute_H_grid (dislocations.py:113-168)
+ * and H_plane (dislocations.py:171-211).                                  */
+int dyf_volterra_h(const double *segs, int nseg, const double *g_vec,
+                   double nu, double a_core, const dyf_volume *vol,
+                   const double *y_values, int ny, int out_prec,
+                   void *H_dev, void *stream);
+
+/* ---- Laue march (laue/kernels.py:48-352, laue/solver.py:34-118) ----- */
+typedef struct {
+    double s0r, s0i, shr, shi, sbr, sbi;  /* sigma_0, sigma_h, sigma_hbar (1/m) */
+    double dz, ds, dx, shift_px;          /* depth step, path step, pitch, shift */
+    double sheet_hw;  /* >0: top-hat half width; <=0: minus Gaussian sigma (m) */
+    double fov_hy;    /* rows with |(iy - Ny/2) dx| >= fov_hy are zero      */
+    int32_t Nz, Nx, Ny, reserved;
+} dyf_laue_args;
+
+/* The 12 closed-form scalars of analytic_line_params (dislocations.py:214-242). */
+typedef struct {
+    double y0, z0, e1y, e1z, e2y, e2z, c_atan, ge1, ge2, b_perp, nu, a2;
+} dyf_line_params;
+
+/* Which kernel path a grid takes, its columns per lane, and the marched
+ * row band [iy_lo, iy_hi) (FoV test evaluated in FP64).                  */
+int dyf_laue_plan(const dyf_laue_args *a, int prec, int *path,
+                  int *cols_per_lane, int *iy_lo, int *iy_hi);
+/* Bytes of the phase buffer dyf_laue_march reads for (mode, prec): the
+ * e^{iH} table on the fast path, 0 on the general path (which reads H).  */
+int dyf_laue_phase_bytes(const dyf_laue_args *a, int mode, int prec,
+                         size_t *bytes);
+/* Bytes of the per-call coefficient workspace for n_e deviations.        */
+int dyf_laue_coef_bytes(const dyf_laue_args *a, int n_e, size_t *bytes);
+/* Fast path: e^{iH} table from an H grid (Ny, Nz, Nx), H_prec = DYF_FP64
+ * (double) or DYF_FP32 (float).  The H_gpu= path of solve_dislocation.    */
+int dyf_laue_phase_from_h(const dyf_laue_args *a, int prec, const void *H_dev,
+                          int H_prec, void *phase_dev, void *stream);
+/* Fast path: e^{iH} table evaluated from segments with no H grid stored
+ * (FP64 Volterra, 2 pi reduction in FP64 for the FP32/mixed table).       */
+int dyf_laue_phase_from_segments(const dyf_laue_args *a, int prec,
+                                 const double *segs, int nseg,
+                                 const double *g_vec, double nu,
+                                 double a_core, void *phase_dev, void *stream);
+/* Fast path, analytic mode: e^{iH(y, z_k)} per marched row and step.     */
+int dyf_laue_phase_analytic(const dyf_laue_args *a, int prec,
+                            const dyf_line_params *line, void *phase_dev,
+                            void *stream);
+/* March n_e deviations (s_dev_dev: n_e doubles on the device) and write
+ * the exit waves out_dev (n_e, Ny, Nx) interleaved complex: complex128 for
+ * DYF_FP64, complex64 otherwise.  `src_dev` is the phase table (fast path)
+ * or the H grid with H_prec (general path, GRID mode); `line` is used in
+ * ANALYTIC mode on the general path.  Replaces the RawKernel launches of
+ * laue/solver.py:73-86 and 107-116.                                       */
+int dyf_laue_march(const dyf_laue_args *a, int mode, int prec,
+                   const void *src_dev, int H_prec,
+                   const dyf_line_params *line, const double *s_dev_dev,
+                   int n_e, void *coef_dev, void *out_dev, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DYFOX_H */

@@ -1,0 +1,69 @@
+"""Shared geometry builders and oracle adapters for the tests."""
+
+import numpy as np
+
+from dyfox_amd.dislocations import burgers_vector, straight_segment
+
+
+def geometry_n(xtal, grid, sign=+1, y0=0.0):
+    """The inclined [101] edge line of the paper geometry, at mid-depth."""
+    xi = np.array([1.0, 0.0, 1.0]) / np.sqrt(2.0)
+    b = burgers_vector(np.eye(3), [1, 0, -1], xtal.b_mag, sign)
+    z0 = grid.t_crystal / 2
+    return straight_segment(xi, b, [-z0 * xtal.tan_tB, y0, z0], 400e-6)
+
+
+def mixed_on_x(xtal, grid, sign=+1):
+    """A mixed line along x_lab with b = a/2[110] (g.b = 2), core half a
+    pixel off the sampled rows: the closed-form geometry."""
+    b = burgers_vector(np.eye(3), [1, 1, 0], xtal.b_mag, sign)
+    return straight_segment([1.0, 0.0, 0.0], b,
+                            [0.0, grid.dx / 2, grid.t_crystal / 2], 400e-6)
+
+
+def grid_args(grid, xtal):
+    Nx, Ny, dz, ds, spx = grid.geometry()
+    return dict(sig0=complex(xtal.sig0), sig_h=complex(xtal.sig_h),
+                sig_hbar=complex(xtal.sig_hbar), Nz=grid.Nz, Nx=Nx, Ny=Ny,
+                dx=grid.dx, dz=dz, ds=ds, shift_px=spx,
+                sheet_hw=grid.sheet_arg(), fov_hy=grid.beam_width / 2.0)
+
+
+def oracle_exit(seg_list, s_dev, grid, xtal, H=None):
+    """Oracle exit waves; H from the oracle Volterra unless given."""
+    from oracle import laue as OL, volterra as OV
+    g = grid_args(grid, xtal)
+    rows = OL.marched_rows(g["Ny"], g["dx"], g["fov_hy"])
+    if H is None and seg_list:
+        H = OV.h_grid(seg_list, xtal.g_vec, xtal.nu, xtal.a_core, g["Nx"],
+                      g["Ny"], grid.Nz, g["dx"], g["dz"], rows=rows)
+    return OL.march_grid(g, s_dev, H=H, rows=rows)
+
+
+def random_segments(rng, xtal, grid, n):
+    """n segments: lines from the 12 <110> directions, b = +-a/2<110>
+    drawn independently, cores uniform in the illuminated fan and half a
+    pixel off the sampled rows, long enough to cross the grid."""
+    dirs = []
+    for i in range(3):
+        for j in range(i + 1, 3):
+            for s in (1, -1):
+                v = np.zeros(3)
+                v[i], v[j] = 1.0, s
+                dirs.append(v / np.sqrt(2))
+                dirs.append(-v / np.sqrt(2))
+    dirs = np.array(dirs)
+    Ny = grid.geometry()[1]
+    segs = []
+    for _ in range(n):
+        xi = dirs[rng.integers(len(dirs))]
+        bh = dirs[rng.integers(len(dirs))]
+        sign = 1 if rng.random() < 0.5 else -1
+        b = sign * xtal.b_mag * bh
+        z = rng.uniform(0, grid.t_crystal)
+        xw = z * xtal.tan_tB
+        x = rng.uniform(-xw, xw)
+        yext = min(Ny * grid.dx / 2, grid.beam_width / 2)
+        y = (np.floor(rng.uniform(-yext, yext) / grid.dx) + 0.5) * grid.dx
+        segs += straight_segment(xi, b, [x, y, z], 1e-3)
+    return segs
