@@ -1,0 +1,232 @@
+"""Benchmark of the Laue hot path on one MI355X; prints ONE JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Workload: the mixed-precision rocking-scan configuration - diamond (400)
+at 17 keV, t = 273 um, dx = 0.25 um, beam 120 um, 256 rows (Nz 490,
+Nx 1056), 64 s_dev uniform in [-200, +200] urad, 8 seeded <110>
+segments: 8.460e9 pixel-steps per solve.  One step is one full
+solve_dislocation call (Volterra e^{iH} table + march of all 64 angles)
+with the inputs already on the host and nothing cached between steps
+but the workspace buffers.
+
+`value` is FP64 pixel-steps/s; the same line carries the mixed
+(two-float FP32) rate on the same workload with its error against FP64,
+the worked-example drop-in rate, the roofline of the dominant kernel
+(the march, HIP events on the stream it runs on) and the CPU oracle's
+rate on a bounded sample of the same workload.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("Laue pixel-steps/s FP64 vs mixed @<=1e-6 rel err; "
+          "Bragg solves/s cold vs warm")
+FLOPS_PER_PIXEL_STEP = 40        # 2x2 complex matvec 28 + two couplings 12
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true",
+                    help="skip the CPU oracle baseline")
+    ap.add_argument("--probe", action="store_true",
+                    help="for counter runs: one phase build and one march "
+                         "per precision, then a 1 GiB copy kernel (the "
+                         "byte calibration), and exit")
+    return ap.parse_args()
+
+
+def laue_step(N, prepare_phase, march, segs, s_dev_dev, grid, xtal, prec,
+              Ny, Nx, ev):
+    """One solve_dislocation, with events around the march launch."""
+    ev[0].record()
+    src = prepare_phase(segs, grid, xtal, precision=prec, reuse=True)
+    out = N.DeviceArray.empty((len(s_dev_dev), Ny, Nx), prec.complex)
+    ev[1].record()
+    march(src, s_dev_dev, grid, xtal, out, prec)
+    ev[2].record()
+    return out
+
+
+def time_laue(N, segs, s, grid, xtal, prec, steps, warmup):
+    from dyfox_amd.laue.solver import march, prepare_phase
+    Nx, Ny = grid.geometry()[:2]
+    sd = N.DeviceArray.from_host(np.asarray(s, np.float64))
+    evs = [[N.Event() for _ in range(3)] for _ in range(steps)]
+    for _ in range(warmup):
+        laue_step(N, prepare_phase, march, segs, sd, grid, xtal, prec, Ny,
+                  Nx, evs[0])
+    N.synchronize()
+    t0 = time.perf_counter()
+    out = None
+    for k in range(steps):
+        out = laue_step(N, prepare_phase, march, segs, sd, grid, xtal, prec,
+                        Ny, Nx, evs[k])
+    N.synchronize()
+    wall = (time.perf_counter() - t0) / steps
+    phase_ms = float(np.mean([e[0].elapsed_ms(e[1]) for e in evs]))
+    march_ms = float(np.mean([e[1].elapsed_ms(e[2]) for e in evs]))
+    return wall, phase_ms, march_ms, out
+
+
+def cpu_baseline(segs, s, grid, xtal, rows=48, n_s=8):
+    """The NumPy FP64 oracle (Volterra H + direct-form march) on a bounded
+    sample: `rows` marched rows x `n_s` angles of the same workload."""
+    from oracle import laue as OL, volterra as OV
+    Nx, Ny, dz, ds, spx = grid.geometry()
+    all_rows = grid.marched_rows()
+    pick = all_rows[np.linspace(0, len(all_rows) - 1, rows).astype(int)]
+    ss = np.asarray(s)[np.linspace(0, len(s) - 1, n_s).astype(int)]
+    t0 = time.perf_counter()
+    H = OV.h_grid(segs, xtal.g_vec, xtal.nu, xtal.a_core, Nx, Ny, grid.Nz,
+                  grid.dx, dz, rows=pick)
+    OL.march(ss, complex(xtal.sig0), complex(xtal.sig_h),
+             complex(xtal.sig_hbar), grid.Nz, Nx, Ny, grid.dx, ds, spx,
+             grid.sheet_arg(), grid.beam_width / 2, H=H, rows=pick)
+    dt = time.perf_counter() - t0
+    work = rows * n_s * Nx * (grid.Nz - 1)
+    return {"value": work / dt, "unit": "pixel-steps/s", "cores": 1,
+            "kind": "port",
+            "sample": f"{rows} marched rows x {n_s} s_dev of the same "
+                      f"workload ({work:.3e} pixel-steps, Volterra H of "
+                      f"those rows included), {dt:.1f} s, 1 process"}
+
+
+def traffic_from_profile():
+    """HBM bytes per march launch from the committed PMC summary."""
+    path = os.path.join(ROOT, "profiles", "laue_march_cfg3_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("fp64_bytes_per_launch")
+
+
+def probe():
+    """One dispatch of each kernel for rocprofv3 --pmc passes."""
+    import ctypes as C
+    from dyfox_amd import _native as N
+    from dyfox_amd.backend import FP64, MIXED, device
+    from dyfox_amd.crystal import DIAMOND, CrystalParams
+    from dyfox_amd.laue.solver import march, prepare_phase
+    from dyfox_amd.workloads import laue_rocking_scan
+    device()
+    xtal = CrystalParams.from_reflection(DIAMOND, (4, 0, 0), 17.0,
+                                         f_prime=0.003, f_dprime=0.002)
+    grid, segs, s = laue_rocking_scan(xtal)
+    Nx, Ny = grid.geometry()[:2]
+    sd = N.DeviceArray.from_host(np.asarray(s, np.float64))
+    for prec in (FP64, MIXED):
+        src = prepare_phase(segs, grid, xtal, precision=prec)
+        out = N.DeviceArray.empty((len(s), Ny, Nx), prec.complex)
+        march(src, sd, grid, xtal, out, prec)
+        N.synchronize()
+        del src, out
+    nb = 1 << 30
+    a = N.DeviceArray.empty((nb,), np.uint8)
+    b = N.DeviceArray.empty((nb,), np.uint8)
+    N.call("dyf_copy_kernel", C.c_void_p(b.ptr), C.c_void_p(a.ptr),
+           C.c_size_t(nb), None)
+    N.synchronize()
+    print("probe done", flush=True)
+
+
+def main():
+    args = parse()
+    if args.probe:
+        return probe()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(rows x s_dev shard with no communication; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+    from dyfox_amd import _native as N
+    from dyfox_amd.backend import FP64, MIXED, device
+    from dyfox_amd.crystal import DIAMOND, CrystalParams
+    from dyfox_amd.workloads import laue_rocking_scan, laue_worked_example
+    from oracle.laue import error_report
+
+    device()
+    xtal = CrystalParams.from_reflection(DIAMOND, (4, 0, 0), 17.0,
+                                         f_prime=0.003, f_dprime=0.002)
+    grid, segs, s = laue_rocking_scan(xtal)
+    work = grid.pixel_steps(len(s))
+    steps, warm = max(1, args.steps), max(0, args.warmup)
+    print(f"[bench] cfg3 {grid} n_s={len(s)} n_seg={len(segs)} "
+          f"work={work:.3e} pixel-steps", file=sys.stderr, flush=True)
+
+    w64, p64, m64, out64 = time_laue(N, segs, s, grid, xtal, FP64, steps,
+                                     warm)
+    print(f"[bench] fp64  step {w64 * 1e3:.2f} ms (phase {p64:.2f}, march "
+          f"{m64:.2f})", file=sys.stderr, flush=True)
+    wmx, pmx, mmx, outmx = time_laue(N, segs, s, grid, xtal, MIXED, steps,
+                                     warm)
+    print(f"[bench] mixed step {wmx * 1e3:.2f} ms (phase {pmx:.2f}, march "
+          f"{mmx:.2f})", file=sys.stderr, flush=True)
+    rep = error_report(outmx.get(), out64.get())
+    worst = {k: max(r[k] for r in rep) for k in rep[0]}
+    del out64, outmx
+
+    g2, seg2, s2 = laue_worked_example(xtal)
+    w2, p2, m2, _ = time_laue(N, seg2, s2, g2, xtal, FP64, max(3, steps // 2),
+                              1)
+    work2 = g2.pixel_steps(len(s2))
+    print(f"[bench] cfg2 fp64 step {w2 * 1e3:.2f} ms", file=sys.stderr,
+          flush=True)
+
+    achieved = FLOPS_PER_PIXEL_STEP * work / (m64 * 1e-3) / 1e12
+    roof = {"kernel": "laue_fast_kernel<C=17, GRID, FP64>",
+            "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
+            "traffic": traffic_from_profile(),
+            "note": "FP64 vector-ALU bound (no MFMA: the march is not a "
+                    "contraction); MI355X's FP64 vector and FP64 matrix "
+                    "peaks are both 78.6 TFLOP/s. 40 algorithmic flops per "
+                    "pixel-step; the kernel itself issues 16 FP64 FMAs",
+            "avg_launch_ms": m64}
+    line = {
+        "metric": METRIC, "value": work / w64, "unit": "pixel-steps/s",
+        "n_gpus": 1, "steps": steps, "warmup": warm,
+        "ms_per_step": w64 * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded 8-segment <110> dislocation field)",
+        "config": {"workload": "laue_rocking_scan_cfg3",
+                   "crystal": "diamond(400) 17 keV", "t_um": 273,
+                   "dx_um": 0.25, "Ny": 256, "Nx": 1056, "Nz": grid.Nz,
+                   "n_s_dev": len(s), "n_segments": len(segs),
+                   "pixel_steps": work, "parallelism": "single GPU"},
+        "fp64": {"value": work / w64, "ms_per_step": w64 * 1e3,
+                 "phase_ms": p64, "march_ms": m64,
+                 "march_only": work / (m64 * 1e-3)},
+        "mixed": {"value": work / wmx, "ms_per_step": wmx * 1e3,
+                  "phase_ms": pmx, "march_ms": mmx,
+                  "march_only": work / (mmx * 1e-3),
+                  "speedup_vs_fp64": w64 / wmx,
+                  "march_speedup_vs_fp64": m64 / mmx,
+                  "err_vs_fp64_worst_s_dev": worst},
+        "worked_example_fp64": {"value": work2 / w2, "ms_per_step": w2 * 1e3,
+                                "phase_ms": p2, "march_ms": m2,
+                                "pixel_steps": work2},
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(segs, s, grid, xtal)
+        line["cpu_baseline"]["gpu_over_cpu"] = line["value"] / \
+            line["cpu_baseline"]["value"]
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

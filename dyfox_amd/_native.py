@@ -59,6 +59,11 @@ _SIGNATURES = {
     "dyf_mem_info": [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)],
     "dyf_synchronize": [],
     "dyf_stream_sync": [vp],
+    "dyf_event_create": [C.POINTER(vp)],
+    "dyf_event_record": [vp, vp],
+    "dyf_event_elapsed_ms": [vp, vp, C.POINTER(C.c_float)],
+    "dyf_event_destroy": [vp],
+    "dyf_copy_kernel": [vp, vp, C.c_size_t, vp],
     "dyf_volterra_h": [dp, C.c_int, dp, C.c_double, C.c_double,
                        C.POINTER(VolumeDesc), dp, C.c_int, C.c_int, vp, vp],
     "dyf_laue_plan": [C.POINTER(LaueArgs), C.c_int, C.POINTER(C.c_int),
@@ -160,6 +165,30 @@ def mem_info():
 
 def synchronize():
     call("dyf_synchronize")
+
+
+class Event:
+    """A hipEvent recorded on the library's (default) stream."""
+
+    def __init__(self):
+        p = vp()
+        call("dyf_event_create", C.byref(p))
+        self._e = p.value
+
+    def record(self, stream=None):
+        call("dyf_event_record", vp(self._e), stream)
+        return self
+
+    def elapsed_ms(self, later):
+        """Milliseconds from this event to `later` (waits for `later`)."""
+        ms = C.c_float()
+        call("dyf_event_elapsed_ms", vp(self._e), vp(later._e), C.byref(ms))
+        return float(ms.value)
+
+    def __del__(self):
+        if getattr(self, "_e", None) and _lib is not None:
+            _lib.dyf_event_destroy(vp(self._e))
+            self._e = None
 
 
 def dptr(a):

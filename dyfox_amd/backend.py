@@ -39,8 +39,13 @@ def array_module(a):
 
 
 def free_gpu():
-    """Device memory is released as arrays are collected; nothing pooled."""
+    """Release the solvers' cached workspaces; arrays free themselves when
+    collected."""
     import gc
+    import sys
+    mod = sys.modules.get(__package__ + ".laue.solver")
+    if mod is not None:
+        mod.release_workspaces()
     gc.collect()
 
 

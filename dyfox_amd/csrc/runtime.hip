@@ -100,6 +100,14 @@ RowBand laue_rows(const dyf_laue_args *a) {
     return band;
 }
 
+__global__ __launch_bounds__(256) void copy16_kernel(uint4 *__restrict__ dst,
+                                                     const uint4 *__restrict__ src,
+                                                     size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * 256)
+        dst[i] = src[i];
+}
+
 }  // namespace dyf
 
 using namespace dyf;
@@ -182,6 +190,43 @@ int dyf_synchronize(void) {
 
 int dyf_stream_sync(void *stream) {
     DYF_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return DYF_OK;
+}
+
+int dyf_copy_kernel(void *dst, const void *src, size_t bytes, void *stream) {
+    if ((bytes & 15) || ((uintptr_t)dst & 15) || ((uintptr_t)src & 15))
+        return fail(DYF_EINVAL, "copy needs 16-byte sizes and alignment");
+    const size_t n = bytes / 16;
+    if (n == 0) return DYF_OK;
+    const int blocks = (int)std::min<size_t>((n + 255) / 256, 8192);
+    copy16_kernel<<<blocks, 256, 0, (hipStream_t)stream>>>(
+        (uint4 *)dst, (const uint4 *)src, n);
+    DYF_LAUNCHED("copy16_kernel");
+    return DYF_OK;
+}
+
+int dyf_event_create(void **event) {
+    if (!event) return fail(DYF_EINVAL, "event is NULL");
+    hipEvent_t e;
+    DYF_HIP(hipEventCreate(&e));
+    *event = (void *)e;
+    return DYF_OK;
+}
+
+int dyf_event_record(void *event, void *stream) {
+    DYF_HIP(hipEventRecord((hipEvent_t)event, (hipStream_t)stream));
+    return DYF_OK;
+}
+
+int dyf_event_elapsed_ms(void *start, void *stop, float *ms) {
+    if (!ms) return fail(DYF_EINVAL, "ms is NULL");
+    DYF_HIP(hipEventSynchronize((hipEvent_t)stop));
+    DYF_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+    return DYF_OK;
+}
+
+int dyf_event_destroy(void *event) {
+    if (event) DYF_HIP(hipEventDestroy((hipEvent_t)event));
     return DYF_OK;
 }
 

@@ -75,8 +75,33 @@ def _guard(nbytes, what):
                           f"{free / 1e9:.1f} GB free")
 
 
-def prepare_phase(seg_list, grid, xtal, *, H_gpu=None, precision=FP64):
-    """Build the phase source a solve marches through (see module doc)."""
+_table_cache = {}
+
+
+def _table(nbytes, reuse):
+    """A byte buffer for the e^{iH} table: the module's grow-only cached
+    buffer when `reuse` (one solve at a time on the default stream, so
+    the previous user has finished reading it), else a fresh one."""
+    if not reuse:
+        return N.DeviceArray.empty((max(nbytes, 1),), np.uint8)
+    buf = _table_cache.get("phase")
+    if buf is None or buf.nbytes < nbytes:
+        _table_cache.pop("phase", None)
+        buf = N.DeviceArray.empty((max(nbytes, 1),), np.uint8)
+        _table_cache["phase"] = buf
+    return buf
+
+
+def release_workspaces():
+    """Drop the cached phase table and coefficient workspace."""
+    _table_cache.clear()
+    _coef_cache.clear()
+
+
+def prepare_phase(seg_list, grid, xtal, *, H_gpu=None, precision=FP64,
+                  reuse=False):
+    """Build the phase source a solve marches through (see module doc).
+    `reuse` writes the e^{iH} table into the module's cached buffer."""
     prec = _precision(precision)
     a = laue_args(grid, xtal)
     path, _, _, _ = plan(grid, xtal, prec)
@@ -96,7 +121,7 @@ def prepare_phase(seg_list, grid, xtal, *, H_gpu=None, precision=FP64):
         N.call("dyf_laue_phase_bytes", C.byref(a), N.MODE_GRID, prec.code,
                C.byref(nb))
         _guard(nb.value, "the e^{iH} table")
-        E = N.DeviceArray.empty((nb.value,), np.uint8)
+        E = _table(nb.value, reuse)
         N.call("dyf_laue_phase_from_h", C.byref(a), prec.code,
                C.c_void_p(H.ptr), h_prec, C.c_void_p(E.ptr), None)
         return PhaseSource(N.MODE_GRID, E)
@@ -107,7 +132,7 @@ def prepare_phase(seg_list, grid, xtal, *, H_gpu=None, precision=FP64):
         nb = C.c_size_t()
         N.call("dyf_laue_phase_bytes", C.byref(a), N.MODE_ANALYTIC, prec.code,
                C.byref(nb))
-        E = N.DeviceArray.empty((nb.value,), np.uint8)
+        E = _table(nb.value, reuse)
         N.call("dyf_laue_phase_analytic", C.byref(a), prec.code,
                C.byref(line), C.c_void_p(E.ptr), None)
         return PhaseSource(N.MODE_ANALYTIC, E, line=line)
@@ -121,7 +146,7 @@ def prepare_phase(seg_list, grid, xtal, *, H_gpu=None, precision=FP64):
     N.call("dyf_laue_phase_bytes", C.byref(a), N.MODE_GRID, prec.code,
            C.byref(nb))
     _guard(nb.value, "the e^{iH} table")
-    E = N.DeviceArray.empty((nb.value,), np.uint8)
+    E = _table(nb.value, reuse)
     segs = pack_segments(seg_list)
     g = np.ascontiguousarray(xtal.g_vec, dtype=np.float64)
     N.call("dyf_laue_phase_from_segments", C.byref(a), prec.code,
@@ -170,7 +195,8 @@ def solve_dislocation(seg_list, s_dev_arr, grid, xtal, *, H_gpu=None,
     s_dev_arr = np.atleast_1d(np.asarray(s_dev_arr, dtype=float))
     n_e = len(s_dev_arr)
     Nx, Ny = grid.geometry()[:2]
-    source = prepare_phase(seg_list, grid, xtal, H_gpu=H_gpu, precision=prec)
+    source = prepare_phase(seg_list, grid, xtal, H_gpu=H_gpu, precision=prec,
+                           reuse=True)
     out = N.DeviceArray.empty((n_e, Ny, Nx), prec.complex)
     sd = N.DeviceArray.from_host(s_dev_arr.astype(np.float64))
     step = max(1, int(batch))

@@ -62,6 +62,18 @@ int dyf_memset(void *ptr_dev, int value, size_t bytes);
 int dyf_mem_info(size_t *free_bytes, size_t *total_bytes);
 int dyf_synchronize(void);
 int dyf_stream_sync(void *stream);
+/* Device timers on a stream (hipEvent): kernels of this library are
+ * timed on the stream they are launched on.                              */
+int dyf_event_create(void **event);
+int dyf_event_record(void *event, void *stream);
+int dyf_event_elapsed_ms(void *start, void *stop, float *ms);
+int dyf_event_destroy(void *event);
+/* Device-to-device copy by a kernel (16-byte vector loads/stores), usable
+ * under graph capture and as the byte-count calibration for HBM traffic
+ * counters.  `bytes` must be a multiple of 16 and both pointers 16-byte
+ * aligned.  No counterpart in the reference.                              */
+int dyf_copy_kernel(void *dst_dev, const void *src_dev, size_t bytes,
+                    void *stream);
 
 /* ---- Volterra displacement phase ------------------------------------ */
 /* One straight dislocation segment, lab frame, metres: ra[3], rb[3], b[3]

@@ -41,29 +41,6 @@ def oracle_exit(seg_list, s_dev, grid, xtal, H=None):
 
 
 def random_segments(rng, xtal, grid, n):
-    """n segments: lines from the 12 <110> directions, b = +-a/2<110>
-    drawn independently, cores uniform in the illuminated fan and half a
-    pixel off the sampled rows, long enough to cross the grid."""
-    dirs = []
-    for i in range(3):
-        for j in range(i + 1, 3):
-            for s in (1, -1):
-                v = np.zeros(3)
-                v[i], v[j] = 1.0, s
-                dirs.append(v / np.sqrt(2))
-                dirs.append(-v / np.sqrt(2))
-    dirs = np.array(dirs)
-    Ny = grid.geometry()[1]
-    segs = []
-    for _ in range(n):
-        xi = dirs[rng.integers(len(dirs))]
-        bh = dirs[rng.integers(len(dirs))]
-        sign = 1 if rng.random() < 0.5 else -1
-        b = sign * xtal.b_mag * bh
-        z = rng.uniform(0, grid.t_crystal)
-        xw = z * xtal.tan_tB
-        x = rng.uniform(-xw, xw)
-        yext = min(Ny * grid.dx / 2, grid.beam_width / 2)
-        y = (np.floor(rng.uniform(-yext, yext) / grid.dx) + 0.5) * grid.dx
-        segs += straight_segment(xi, b, [x, y, z], 1e-3)
-    return segs
+    """The benchmark's <110>-family segment draw (dyfox_amd.workloads)."""
+    from dyfox_amd.workloads import random_segments as draw
+    return draw(rng, xtal, grid, n)
