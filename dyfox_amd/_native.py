@@ -47,6 +47,22 @@ class LineParams(C.Structure):
                  "ge2", "b_perp", "nu", "a2")]
 
 
+class BraggArgs(C.Structure):
+    _fields_ = [(n, C.c_double) for n in
+                ("s0r", "s0i", "shr", "shi", "sbr", "sbi", "ds")] + \
+               [(n, C.c_int32) for n in ("Nz", "Nx", "nyc", "reserved")]
+
+
+class GmresOpts(C.Structure):
+    _fields_ = [("tol", C.c_double), ("restart", C.c_int32),
+                ("maxiter", C.c_int32)]
+
+
+class GmresInfo(C.Structure):
+    _fields_ = [("residual", C.c_double), ("iterations", C.c_int32),
+                ("code", C.c_int32)]
+
+
 _SIGNATURES = {
     "dyf_init": [C.c_int],
     "dyf_device_count": [C.POINTER(C.c_int)],
@@ -82,6 +98,18 @@ _SIGNATURES = {
                                 C.POINTER(LineParams), vp, vp],
     "dyf_laue_march": [C.POINTER(LaueArgs), C.c_int, C.c_int, vp, C.c_int,
                        C.POINTER(LineParams), vp, C.c_int, vp, vp, vp],
+    "dyf_bragg_phase": [dp, C.c_int, dp, C.c_double, C.c_double,
+                        C.POINTER(VolumeDesc), dp, C.c_int, vp, vp],
+    "dyf_bragg_march": [C.POINTER(BraggArgs), vp, C.c_int, vp, vp, vp, vp,
+                        vp],
+    "dyf_bragg_precondition": [C.POINTER(BraggArgs), vp, C.c_int, vp, vp, vp,
+                               vp],
+    "dyf_bragg_workspace_bytes": [C.POINTER(BraggArgs), C.c_int, C.c_int,
+                                  C.POINTER(C.c_size_t)],
+    "dyf_bragg_solve": [C.POINTER(BraggArgs), vp, C.c_int, vp, vp,
+                        C.POINTER(GmresOpts), vp, vp, vp, C.c_size_t,
+                        C.POINTER(GmresInfo), vp],
+    "dyf_bragg_exit_wave": [C.POINTER(BraggArgs), C.c_int, vp, vp, vp, vp],
 }
 
 _lib = None

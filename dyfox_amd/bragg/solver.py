@@ -1,0 +1,271 @@
+"""The Bragg boundary-value problem, solved on the device by shooting.
+
+    surface, info = solve_surface_batch(segs, s_dev, cfg, ys, xtal, g)
+    wave, info    = solve_bragg_stable(segs, s_dev, cfg, xtal, g)
+
+Both amplitudes are marched down from the surface with D_g(0) = v
+unknown; the back-face condition D_g(t) = 0 gives A v = -d, A v the
+homogeneous response and d the particular one, solved by restarted GMRES
+left-preconditioned with the perfect crystal's Fourier symbol.  The
+march, the preconditioner and the whole Krylov process run in
+libdyfox.so; this module carries the reference's measured-residual policy
+(bragg/solver.py:182-379): a chunk whose true residual exceeds `accept`
+is retried warm-started with a Krylov space `grow` times larger, up to
+`tries` attempts or `restart_cap`, then split in halves down to
+`max_depth`.
+
+Superset of the reference API: `s_dev` may be an array, in which case
+every angle of a chunk is solved in the same batched GMRES (one system
+per angle, lockstep iterations) and the result is (n_s, ny, Nx) with a
+list of per-angle `info` dicts; `warm_start=` supplies initial v guesses
+(the learned warm start), shape (ny, Nx) or (n_s, ny, Nx).
+"""
+
+import ctypes as C
+
+import numpy as np
+
+from .. import _native as N
+from ..dislocations import pack_segments
+
+
+def perfect_symbol(e0, eg, ds, Nz, Nx, xtal):
+    """T21, T22 of the unstrained shooting chain, mode by mode (host):
+    D_g(t) = T21 D_0(0) + T22 D_g(0) per spatial frequency."""
+    sh, sb = complex(xtal.sig_h), complex(xtal.sig_hbar)
+    m = np.fft.fftfreq(Nx) * Nx
+    p = np.exp(-2j * np.pi * m / Nx)
+    pi = np.conj(p)
+    T11 = np.ones(Nx, complex)
+    T12 = np.zeros(Nx, complex)
+    T21 = np.zeros(Nx, complex)
+    T22 = np.ones(Nx, complex)
+    for _ in range(Nz - 1):
+        a11, a12 = e0 * p, e0 * p * ds * sb
+        a21, a22 = -pi * ds * sh / eg, pi / eg
+        T11, T12, T21, T22 = (a11 * T11 + a12 * T21, a11 * T12 + a12 * T22,
+                              a21 * T11 + a22 * T21, a21 * T12 + a22 * T22)
+    return T21, T22
+
+
+def bragg_args(cfg, xtal, nyc):
+    return N.BraggArgs(
+        s0r=float(np.real(xtal.sig0)), s0i=float(np.imag(xtal.sig0)),
+        shr=float(np.real(xtal.sig_h)), shi=float(np.imag(xtal.sig_h)),
+        sbr=float(np.real(xtal.sig_hbar)), sbi=float(np.imag(xtal.sig_hbar)),
+        ds=float(cfg.ds), Nz=int(cfg.Nz), Nx=int(cfg.Nx), nyc=int(nyc),
+        reserved=0)
+
+
+def chunk_phase(seg_list, cfg, xtal, g_vec, y_values):
+    """e^{iH} on the planes y_values: device (Nz, ny, Nx) complex128."""
+    ys = np.ascontiguousarray(y_values, dtype=np.float64)
+    E = N.DeviceArray.empty((cfg.Nz, len(ys), cfg.Nx), np.complex128)
+    segs = pack_segments(seg_list)
+    g = np.ascontiguousarray(g_vec, dtype=np.float64)
+    vol = N.VolumeDesc(int(cfg.Nx), int(cfg.Nz), float(cfg.dx), float(cfg.dz))
+    N.call("dyf_bragg_phase", N.dptr(segs), len(segs), N.dptr(g),
+           float(xtal.nu), float(xtal.a_core), C.byref(vol), N.dptr(ys),
+           len(ys), C.c_void_p(E.ptr), None)
+    return E
+
+
+def march(cfg, xtal, s_dev, E, incident=None, Dg0=None):
+    """Dg(t) of the down-march, one system per s_dev: (n_s, nyc, Nx)."""
+    s = np.atleast_1d(np.asarray(s_dev, np.float64))
+    nyc = E.shape[1]
+    a = bragg_args(cfg, xtal, nyc)
+    sd = N.DeviceArray.from_host(s)
+    inc = None if incident is None else N.asarray(
+        np.asarray(incident, np.complex128).reshape(cfg.Nx))
+    v = None if Dg0 is None else N.asarray(
+        np.asarray(Dg0, np.complex128).reshape(len(s), nyc, cfg.Nx))
+    out = N.DeviceArray.empty((len(s), nyc, cfg.Nx), np.complex128)
+    N.call("dyf_bragg_march", C.byref(a), C.c_void_p(sd.ptr), len(s),
+           C.c_void_p(E.ptr), None if inc is None else C.c_void_p(inc.ptr),
+           None if v is None else C.c_void_p(v.ptr), C.c_void_p(out.ptr), None)
+    return out
+
+
+def precondition(cfg, xtal, s_dev, v):
+    """M v for each system (host arrays in and out), for tests."""
+    s = np.atleast_1d(np.asarray(s_dev, np.float64))
+    v = np.ascontiguousarray(v, np.complex128)
+    nyc = v.shape[1]
+    a = bragg_args(cfg, xtal, nyc)
+    sd = N.DeviceArray.from_host(s)
+    vd = N.DeviceArray.from_host(v)
+    w = N.DeviceArray.empty(v.shape, np.complex128)
+    pinv = N.DeviceArray.empty((len(s), cfg.Nx), np.complex128)
+    N.call("dyf_bragg_precondition", C.byref(a), C.c_void_p(sd.ptr), len(s),
+           C.c_void_p(vd.ptr), C.c_void_p(w.ptr), C.c_void_p(pinv.ptr), None)
+    return w.get()
+
+
+_ws_cache = {}
+
+
+def _workspace(nbytes):
+    ws = _ws_cache.get("ws")
+    if ws is None or ws.nbytes < nbytes:
+        _ws_cache.pop("ws", None)
+        ws = N.DeviceArray.empty((max(nbytes, 1),), np.uint8)
+        _ws_cache["ws"] = ws
+    return ws
+
+
+def gmres_batch(cfg, xtal, s_dev, E, incident, *, tol, restart, maxiter,
+                x0=None):
+    """One batched GMRES call: v (n_s, nyc, Nx) device and per-system
+    (residual, iterations, code)."""
+    s = np.atleast_1d(np.asarray(s_dev, np.float64))
+    n = len(s)
+    nyc = E.shape[1]
+    a = bragg_args(cfg, xtal, nyc)
+    nb = C.c_size_t()
+    N.call("dyf_bragg_workspace_bytes", C.byref(a), n, int(restart),
+           C.byref(nb))
+    ws = _workspace(nb.value)
+    sd = N.DeviceArray.from_host(s)
+    inc = N.asarray(np.asarray(incident, np.complex128).reshape(cfg.Nx))
+    x = N.DeviceArray.empty((n, nyc, cfg.Nx), np.complex128)
+    x0d = None if x0 is None else N.asarray(x0)
+    opts = N.GmresOpts(tol=float(tol), restart=int(restart),
+                       maxiter=int(maxiter))
+    info = (N.GmresInfo * n)()
+    N.call("dyf_bragg_solve", C.byref(a), C.c_void_p(sd.ptr), n,
+           C.c_void_p(E.ptr), C.c_void_p(inc.ptr), C.byref(opts),
+           None if x0d is None else C.c_void_p(x0d.ptr), C.c_void_p(x.ptr),
+           C.c_void_p(ws.ptr), C.c_size_t(ws.nbytes), info, None)
+    return x, [(info[j].residual, info[j].iterations, info[j].code)
+               for j in range(n)]
+
+
+def exit_wave(cfg, xtal, E, v):
+    """v e^{-iH(z=0)} on the host, (n_s, nyc, Nx)."""
+    a = bragg_args(cfg, xtal, E.shape[1])
+    out = N.DeviceArray.empty(v.shape, np.complex128)
+    N.call("dyf_bragg_exit_wave", C.byref(a), v.shape[0], C.c_void_p(E.ptr),
+           C.c_void_p(v.ptr), C.c_void_p(out.ptr), None)
+    return out.get()
+
+
+def _pick(arr, idx):
+    """Rows `idx` of a device (n, ...) array as a new device array."""
+    host = arr.get()
+    return N.DeviceArray.from_host(host[idx])
+
+
+def solve_surface_batch(seg_list, s_dev, cfg, y_values, xtal, g_vec,
+                        xp=None, chunk=32, tol=1e-10, restart=40,
+                        maxiter=25, verbose=False, gpu_gmres=True,
+                        accept=1e-7, grow=4, tries=3, restart_cap=256,
+                        max_depth=6, per_plane=6, chunk_cap=32,
+                        warm_start=None):
+    """Exit wave on the (ny, Nx) surface grid (see module docstring).
+
+    Returns (out, info): out (ny, Nx) complex128 NumPy for a scalar s_dev
+    and info with the reference's keys ("residual", "iterations",
+    "retries", "splits", "unconverged_chunks", "converged"); for an array
+    of s_dev, out is (n_s, ny, Nx) and info a list of such dicts.
+    `xp` and `gpu_gmres` are accepted for compatibility; every solve runs
+    on the device.
+    """
+    scalar = np.ndim(s_dev) == 0
+    s_all = np.atleast_1d(np.asarray(s_dev, np.float64))
+    n_s = len(s_all)
+    y_values = np.asarray(y_values, np.float64)
+    ny = len(y_values)
+    out = np.zeros((n_s, ny, cfg.Nx), np.complex128)
+    stats = [dict(worst=0.0, its=0, retries=0, splits=0, unconverged=0)
+             for _ in range(n_s)]
+    inc = cfg.incident(n=1)[0]
+    warm = None
+    if warm_start is not None:
+        warm = np.asarray(warm_start, np.complex128)
+        if warm.ndim == 2:
+            warm = np.broadcast_to(warm, (n_s,) + warm.shape)
+        if warm.shape != (n_s, ny, cfg.Nx):
+            raise ValueError(f"warm_start must be (ny, Nx) or (n_s, ny, Nx), "
+                             f"got {warm.shape}")
+    chunk = max(1, min(chunk, chunk_cap))
+
+    def solve_block(sys_idx, j0, j1, depth=0):
+        nyc = j1 - j0
+        E = chunk_phase(seg_list, cfg, xtal, g_vec, y_values[j0:j1])
+        cap = max(restart, min(restart_cap, 8 * nyc))
+        r = max(restart, min(cap, per_plane * nyc))
+        pending = list(sys_idx)
+        x0 = None if warm is None else N.DeviceArray.from_host(
+            np.ascontiguousarray(warm[pending, j0:j1]))
+        res = {}
+        cnt = {s: 0 for s in pending}
+        V = {}
+        for _ in range(tries):
+            v, info = gmres_batch(cfg, xtal, s_all[pending], E, inc, tol=tol,
+                                  restart=r, maxiter=maxiter, x0=x0)
+            vh = v.get()
+            for j, s in enumerate(pending):
+                res[s] = info[j][0]
+                cnt[s] += info[j][1]
+                V[s] = vh[j]
+            retry = [j for j, s in enumerate(pending)
+                     if res[s] > accept and r < cap]
+            if not retry:
+                break
+            for j in retry:
+                stats[pending[j]]["retries"] += 1
+                if verbose:
+                    print(f"      s {pending[j]} y {j0}-{j1}: residual "
+                          f"{res[pending[j]]:.1e} above {accept:.0e}, "
+                          f"retrying with restart {min(r * grow, cap)}",
+                          flush=True)
+            x0 = N.DeviceArray.from_host(
+                np.ascontiguousarray(vh[retry]))
+            pending = [pending[j] for j in retry]
+            r = min(r * grow, cap)
+        split = [s for s in sys_idx
+                 if res[s] > accept and nyc > 1 and depth < max_depth]
+        keep = [s for s in sys_idx if s not in split]
+        if keep:
+            vk = N.DeviceArray.from_host(np.stack([V[s] for s in keep]))
+            ew = exit_wave(cfg, xtal, E, vk)
+            for j, s in enumerate(keep):
+                st = stats[s]
+                st["unconverged"] += res[s] > accept
+                st["worst"] = max(st["worst"], res[s])
+                st["its"] = max(st["its"], cnt[s])
+                out[s, j0:j1] = ew[j]
+                if verbose:
+                    print(f"      s {s} y {j0}-{j1}: {cnt[s]} its, res "
+                          f"{res[s]:.1e}", flush=True)
+        del E
+        if split:
+            for s in split:
+                stats[s]["splits"] += 1
+            mid = j0 + nyc // 2
+            solve_block(split, j0, mid, depth + 1)
+            solve_block(split, mid, j1, depth + 1)
+
+    for j0 in range(0, ny, chunk):
+        solve_block(list(range(n_s)), j0, min(j0 + chunk, ny))
+    infos = [{"residual": st["worst"], "iterations": st["its"],
+              "retries": st["retries"], "splits": st["splits"],
+              "unconverged_chunks": st["unconverged"],
+              "converged": st["unconverged"] == 0} for st in stats]
+    if scalar:
+        return out[0], infos[0]
+    return out, infos
+
+
+def solve_bragg_stable(seg_list, s_dev, cfg, xtal, g_vec, xp=None,
+                       tol=1e-10, y_plane=0.0, restart=40, maxiter=20):
+    """Single-plane shooting solve (bragg/solver.py:135-179): returns the
+    (Nx,) exit wave and {"gmres_code", "iterations", "residual"}."""
+    E = chunk_phase(seg_list, cfg, xtal, g_vec, [float(y_plane)])
+    inc = cfg.incident(n=1)[0]
+    v, info = gmres_batch(cfg, xtal, [float(s_dev)], E, inc, tol=tol,
+                          restart=restart, maxiter=maxiter)
+    ew = exit_wave(cfg, xtal, E, v)[0, 0]
+    res, its, code = info[0]
+    return ew, {"gmres_code": code, "iterations": its, "residual": res}

@@ -1,0 +1,853 @@
+// Bragg (reflection) shooting solve on gfx950: the down-march as a batched
+// matvec, the perfect-crystal Fourier preconditioner, and restarted GMRES
+// whose Arnoldi basis, orthogonalisation and Givens rotations live on the
+// device (reference bragg/solver.py:28-60, 182-379; SciPy 1.15 gmres).
+//
+// A batch is `nsys` independent systems (one s_dev each) over the same
+// chunk of `nyc` y-planes, so they share the e^{iH} table E (Nz, nyc, Nx).
+// Vectors are (nsys, nyc, Nx) complex128, system-major.
+//
+// GMRES follows SciPy's control flow exactly - left preconditioning,
+// inner stop at presid <= ptol, true residual b - A x after every cycle,
+// the adaptive ptol update, maxiter counted in cycles - so a system
+// converges under the same rule and reports the same quantities.  The
+// orthogonalisation is classical Gram-Schmidt applied twice (CGS2): two
+// batched reductions instead of SciPy's col+1 sequential MGS sweeps,
+// equally stable, and the only change is in rounding.
+#include <hipfft/hipfft.h>
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "common.hpp"
+
+namespace dyf {
+
+using cd = cx<double>;
+
+__device__ __forceinline__ cd cmul(cd a, cd b) {
+    return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r};
+}
+__device__ __forceinline__ cd cconjmul(cd a, cd b) {   // conj(a) * b
+    return {a.r * b.r + a.i * b.i, a.r * b.i - a.i * b.r};
+}
+__device__ __forceinline__ cd cadd(cd a, cd b) { return {a.r + b.r, a.i + b.i}; }
+__device__ __forceinline__ cd cscale(cd a, double s) { return {a.r * s, a.i * s}; }
+__device__ __forceinline__ cd cexp(double re, double im) {
+    double s, c;
+    sincos(im, &s, &c);
+    const double m = exp(re);
+    return {m * c, m * s};
+}
+__device__ __forceinline__ cd cdiv(cd a, cd b) {
+    const double d = b.r * b.r + b.i * b.i;
+    return {(a.r * b.r + a.i * b.i) / d, (a.i * b.r - a.r * b.i) / d};
+}
+
+struct BraggK {
+    double s0r, s0i, shr, shi, sbr, sbi, ds;
+    int Nz, Nx, nyc, nsys;
+};
+
+static BraggK make_k(const dyf_bragg_args *a, int nsys) {
+    BraggK k;
+    k.s0r = a->s0r, k.s0i = a->s0i, k.shr = a->shr, k.shi = a->shi;
+    k.sbr = a->sbr, k.sbi = a->sbi, k.ds = a->ds;
+    k.Nz = a->Nz, k.Nx = a->Nx, k.nyc = a->nyc, k.nsys = nsys;
+    return k;
+}
+
+// ---- the down-march: one workgroup per (system, plane) ----------------
+// Each thread owns source columns x; per step it forms
+//   p = D0[x] + ds sbar e^{+iH} Dg[x]  -> D0'[x+1] = e0 p
+//   q = Dg[x] - ds sh  e^{-iH} D0[x]   -> Dg'[x-1] = q / eg
+// (periodic in x), so every e^{iH} element is read once per step.
+constexpr int kMarchThreads = 256;
+constexpr int kMaxPerThread = 8;          // Nx <= 2048
+
+__global__ __launch_bounds__(kMarchThreads) void bragg_march_kernel(
+    BraggK k, const double *__restrict__ s_dev, const cd *__restrict__ E,
+    const cd *__restrict__ inc, const cd *__restrict__ vin, size_t vin_stride,
+    cd *__restrict__ out, size_t out_stride, double out_scale,
+    const int *__restrict__ active) {
+    extern __shared__ cd lds_b[];
+    const int s = blockIdx.x / k.nyc, j = blockIdx.x % k.nyc;
+    if (active && !active[s]) return;
+    const int Nx = k.Nx, tid = threadIdx.x;
+    cd *D0[2] = {lds_b, lds_b + 2 * Nx};
+    cd *Dg[2] = {lds_b + Nx, lds_b + 3 * Nx};
+    for (int x = tid; x < Nx; x += kMarchThreads) {
+        D0[0][x] = inc ? inc[x] : cd{0.0, 0.0};
+        Dg[0][x] = vin ? vin[(size_t)s * vin_stride + (size_t)j * Nx + x]
+                       : cd{0.0, 0.0};
+    }
+    const double alpha = -2.0 * kPi * s_dev[s];
+    const cd e0 = cexp(k.s0r * k.ds, k.s0i * k.ds);
+    const cd eg = cexp(k.s0r * k.ds, (k.s0i + alpha) * k.ds);
+    const cd ieg = cdiv(cd{1.0, 0.0}, eg);
+    const cd A = {k.ds * k.sbr, k.ds * k.sbi};   // ds sigma_hbar
+    const cd B = {k.ds * k.shr, k.ds * k.shi};   // ds sigma_h
+    __syncthreads();
+    int cur = 0;
+    const size_t plane = (size_t)k.nyc * Nx;
+    cd en[kMaxPerThread];
+#pragma unroll
+    for (int m = 0; m < kMaxPerThread; ++m) {
+        const int x = tid + m * kMarchThreads;
+        if (x < Nx) en[m] = E[(size_t)j * Nx + x];
+    }
+    for (int step = 0; step < k.Nz - 1; ++step) {
+        cd ec[kMaxPerThread];
+#pragma unroll
+        for (int m = 0; m < kMaxPerThread; ++m) ec[m] = en[m];
+        if (step + 1 < k.Nz - 1) {
+            const cd *En = E + (size_t)(step + 1) * plane + (size_t)j * Nx;
+#pragma unroll
+            for (int m = 0; m < kMaxPerThread; ++m) {
+                const int x = tid + m * kMarchThreads;
+                if (x < Nx) en[m] = En[x];
+            }
+        }
+        const cd *d0 = D0[cur], *dg = Dg[cur];
+        cd *n0 = D0[1 - cur], *ng = Dg[1 - cur];
+#pragma unroll
+        for (int m = 0; m < kMaxPerThread; ++m) {
+            const int x = tid + m * kMarchThreads;
+            if (x >= Nx) break;
+            const cd a0 = d0[x], ag = dg[x], e = ec[m];
+            const cd cb = cmul(A, e);          // ds sbar e^{+iH}
+            const cd ch = cconjmul(e, B);      // ds sh  e^{-iH}
+            const cd p = cadd(a0, cmul(cb, ag));
+            const cd nq = cmul(ch, a0);
+            const cd q = {ag.r - nq.r, ag.i - nq.i};
+            n0[x + 1 < Nx ? x + 1 : 0] = cmul(e0, p);
+            ng[x > 0 ? x - 1 : Nx - 1] = cmul(q, ieg);
+        }
+        __syncthreads();
+        cur = 1 - cur;
+    }
+    const cd *dg = Dg[cur];
+    for (int x = tid; x < Nx; x += kMarchThreads)
+        out[(size_t)s * out_stride + (size_t)j * Nx + x] =
+            cscale(dg[x], out_scale);
+}
+
+static int launch_march(const BraggK &k, const double *s_dev, const cd *E,
+                        const cd *inc, const cd *vin, size_t vin_stride,
+                        cd *out, size_t out_stride, double out_scale,
+                        const int *active, hipStream_t stream) {
+    const size_t lds = (size_t)4 * k.Nx * sizeof(cd);
+    static std::once_flag once;
+    std::call_once(once, [] {
+        (void)hipFuncSetAttribute((const void *)bragg_march_kernel,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
+    });
+    hipLaunchKernelGGL(bragg_march_kernel, dim3(k.nsys * k.nyc),
+                       dim3(kMarchThreads), lds, stream, k, s_dev, E, inc,
+                       vin, vin_stride, out, out_stride, out_scale, active);
+    DYF_LAUNCHED("bragg_march_kernel");
+    return DYF_OK;
+}
+
+// ---- perfect-crystal symbol and the preconditioner scale --------------
+// One block per system: T22(m) of the unstrained chain over Nz-1 steps
+// (bragg/solver.py:38-60), floored at 1e-8 max|T22| by a real value
+// (163-165), stored as 1 / (T22s Nx) so the inverse FFT needs no extra pass.
+__global__ __launch_bounds__(256) void bragg_symbol_kernel(
+    BraggK k, const double *__restrict__ s_dev, cd *__restrict__ pinv) {
+    __shared__ double red[256];
+    const int s = blockIdx.x, Nx = k.Nx;
+    const double alpha = -2.0 * kPi * s_dev[s];
+    const cd e0 = cexp(k.s0r * k.ds, k.s0i * k.ds);
+    const cd eg = cexp(k.s0r * k.ds, (k.s0i + alpha) * k.ds);
+    const cd sh = {k.shr, k.shi}, sb = {k.sbr, k.sbi};
+    cd *T = pinv + (size_t)s * Nx;
+    double mx = 0.0;
+    for (int idx = threadIdx.x; idx < Nx; idx += 256) {
+        const int m = idx <= (Nx - 1) / 2 ? idx : idx - Nx;
+        double sn, cs;
+        sincos(-2.0 * kPi * (double)m / (double)Nx, &sn, &cs);
+        const cd p = {cs, sn}, pc = {cs, -sn};
+        const cd a11 = cmul(e0, p);
+        const cd a12 = cmul(cscale(cmul(e0, p), k.ds), sb);
+        const cd a21 = cdiv(cmul(cscale(cd{-pc.r, -pc.i}, k.ds), sh), eg);
+        const cd a22 = cdiv(pc, eg);
+        cd T11 = {1, 0}, T12 = {0, 0}, T21 = {0, 0}, T22 = {1, 0};
+        for (int st = 0; st < k.Nz - 1; ++st) {
+            const cd n11 = cadd(cmul(a11, T11), cmul(a12, T21));
+            const cd n12 = cadd(cmul(a11, T12), cmul(a12, T22));
+            const cd n21 = cadd(cmul(a21, T11), cmul(a22, T21));
+            const cd n22 = cadd(cmul(a21, T12), cmul(a22, T22));
+            T11 = n11, T12 = n12, T21 = n21, T22 = n22;
+        }
+        T[idx] = T22;
+        mx = fmax(mx, sqrt(T22.r * T22.r + T22.i * T22.i));
+    }
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    const double floor_v = 1e-8 * red[0];
+    for (int idx = threadIdx.x; idx < Nx; idx += 256) {
+        cd t = T[idx];
+        if (sqrt(t.r * t.r + t.i * t.i) < floor_v) t = cd{floor_v, 0.0};
+        T[idx] = cdiv(cd{1.0 / (double)Nx, 0.0}, t);
+    }
+}
+
+__global__ __launch_bounds__(256) void spectrum_scale_kernel(
+    cd *__restrict__ w, const cd *__restrict__ pinv, int Nx, int nyc,
+    size_t n_total) {
+    for (size_t t = (size_t)blockIdx.x * 256 + threadIdx.x; t < n_total;
+         t += (size_t)gridDim.x * 256) {
+        const size_t s = t / ((size_t)nyc * Nx);
+        const int m = (int)(t % Nx);
+        w[t] = cmul(w[t], pinv[s * Nx + m]);
+    }
+}
+
+struct FftPlan {
+    hipfftHandle h;
+};
+static std::mutex g_fft_mu;
+static std::map<std::pair<int, int>, FftPlan> g_plans;
+
+static int fft_plan(int Nx, int batch, hipStream_t stream, hipfftHandle *out) {
+    std::lock_guard<std::mutex> lk(g_fft_mu);
+    auto key = std::make_pair(Nx, batch);
+    auto it = g_plans.find(key);
+    if (it == g_plans.end()) {
+        hipfftHandle h;
+        int n = Nx;
+        if (hipfftPlanMany(&h, 1, &n, nullptr, 1, Nx, nullptr, 1, Nx,
+                           HIPFFT_Z2Z, batch) != HIPFFT_SUCCESS)
+            return fail(DYF_EHIP, "hipfftPlanMany(n=%d, batch=%d) failed", Nx,
+                        batch);
+        it = g_plans.emplace(key, FftPlan{h}).first;
+    }
+    if (hipfftSetStream(it->second.h, stream) != HIPFFT_SUCCESS)
+        return fail(DYF_EHIP, "hipfftSetStream failed");
+    *out = it->second.h;
+    return DYF_OK;
+}
+
+// w = M v: forward FFT along x, divide by T22s (times 1/Nx), inverse FFT.
+static int apply_precon(const BraggK &k, const cd *pinv, cd *v, cd *w,
+                        hipStream_t stream) {
+    hipfftHandle h;
+    int rc = fft_plan(k.Nx, k.nsys * k.nyc, stream, &h);
+    if (rc) return rc;
+    if (hipfftExecZ2Z(h, (hipfftDoubleComplex *)v, (hipfftDoubleComplex *)w,
+                      HIPFFT_FORWARD) != HIPFFT_SUCCESS)
+        return fail(DYF_EHIP, "hipfftExecZ2Z forward failed");
+    const size_t n = (size_t)k.nsys * k.nyc * k.Nx;
+    const int blocks = (int)std::min<size_t>((n + 255) / 256, 16384);
+    spectrum_scale_kernel<<<blocks, 256, 0, stream>>>(w, pinv, k.Nx, k.nyc, n);
+    DYF_LAUNCHED("spectrum_scale_kernel");
+    if (hipfftExecZ2Z(h, (hipfftDoubleComplex *)w, (hipfftDoubleComplex *)w,
+                      HIPFFT_BACKWARD) != HIPFFT_SUCCESS)
+        return fail(DYF_EHIP, "hipfftExecZ2Z backward failed");
+    return DYF_OK;
+}
+
+// ---- batched reductions ----------------------------------------------
+// part[(s*kc + q)*P + p] = sum over chunk p of conj(A_sq[i]) * B_s[i]
+constexpr int kRedThreads = 256;
+
+__global__ __launch_bounds__(kRedThreads) void batched_dot_kernel(
+    const cd *__restrict__ A, size_t a_sys, size_t a_vec, int kc,
+    const cd *__restrict__ Bv, size_t b_sys, size_t N, int P,
+    cd *__restrict__ part, const int *__restrict__ active) {
+    __shared__ double rr[kRedThreads], ri[kRedThreads];
+    const int sq = blockIdx.x, p = blockIdx.y;
+    const int s = sq / kc, q = sq % kc;
+    if (active && !active[s]) return;
+    const cd *a = A + s * a_sys + q * a_vec;
+    const cd *b = Bv + s * b_sys;
+    const size_t chunk = (N + P - 1) / P;
+    const size_t lo = (size_t)p * chunk, hi = std::min(N, lo + chunk);
+    double sr = 0.0, si = 0.0;
+    for (size_t i = lo + threadIdx.x; i < hi; i += kRedThreads) {
+        const cd x = a[i], y = b[i];
+        sr += x.r * y.r + x.i * y.i;
+        si += x.r * y.i - x.i * y.r;
+    }
+    rr[threadIdx.x] = sr;
+    ri[threadIdx.x] = si;
+    __syncthreads();
+    for (int w = kRedThreads / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            rr[threadIdx.x] += rr[threadIdx.x + w];
+            ri[threadIdx.x] += ri[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(size_t)sq * P + p] = cd{rr[0], ri[0]};
+}
+
+// out[s*ostride + q] = sum_p part[...] (fixed order: deterministic)
+__global__ void sum_parts_kernel(const cd *__restrict__ part, int P, int n,
+                                 cd *__restrict__ out, int ostride,
+                                 const int *active, int kc) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) return;
+    const int s = t / kc, q = t % kc;
+    if (active && !active[s]) return;
+    double r = 0.0, i = 0.0;
+    for (int p = 0; p < P; ++p)
+        r += part[(size_t)t * P + p].r, i += part[(size_t)t * P + p].i;
+    out[(size_t)s * ostride + q] = cd{r, i};
+}
+
+struct SysState;
+__device__ int sys_cols(const SysState *st, int s);
+
+// y_s[i] += sign * sum_q c_sq * V_sq[i], q < kc (or the system's own
+// column count col+1 when `st` is given)
+__global__ __launch_bounds__(256) void combine_kernel(
+    const cd *__restrict__ V, size_t v_sys, size_t v_vec, int kc,
+    const SysState *__restrict__ st, const cd *__restrict__ coef,
+    size_t c_sys, double sign, cd *__restrict__ y, size_t y_sys, size_t N,
+    const int *__restrict__ active) {
+    const int s = blockIdx.y;
+    if (active && !active[s]) return;
+    if (st) kc = sys_cols(st, s);
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= N) return;
+    const cd *c = coef + s * c_sys;
+    const cd *v = V + s * v_sys + i;
+    double ar = 0.0, ai = 0.0;
+    for (int q = 0; q < kc; ++q) {
+        const cd a = c[q], b = v[(size_t)q * v_vec];
+        ar += a.r * b.r - a.i * b.i;
+        ai += a.r * b.i + a.i * b.r;
+    }
+    cd &o = y[s * y_sys + i];
+    o.r += sign * ar;
+    o.i += sign * ai;
+}
+
+// dst_s = src_s * scale_s  (scale given as 1/norm; skipped if zero)
+__global__ __launch_bounds__(256) void scale_copy_kernel(
+    const cd *__restrict__ src, size_t s_sys, cd *__restrict__ dst,
+    size_t d_sys, const double *__restrict__ inv, size_t N,
+    const int *__restrict__ active) {
+    const int s = blockIdx.y;
+    if (active && !active[s]) return;
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= N) return;
+    dst[s * d_sys + i] = cscale(src[s * s_sys + i], inv[s]);
+}
+
+// r_s = b_s - a_s
+__global__ __launch_bounds__(256) void sub_kernel(const cd *__restrict__ b,
+                                                  const cd *__restrict__ a,
+                                                  cd *__restrict__ r, size_t N,
+                                                  const int *__restrict__ active) {
+    const int s = blockIdx.y;
+    if (active && !active[s]) return;
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= N) return;
+    const size_t at = (size_t)s * N + i;
+    r[at] = cd{b[at].r - a[at].r, b[at].i - a[at].i};
+}
+
+// ---- per-system small dense work (one thread per system) --------------
+struct SysState {
+    double bnrm2, atol, ptol, pmf, presid, rnorm, h0;
+    int col, breakdown, inner_done, done, iters, code, pad_;
+};
+
+__device__ int sys_cols(const SysState *st, int s) { return st[s].col + 1; }
+
+// LAPACK zlartg convention: [c s; -conj(s) c] [f; g] = [r; 0], c real.
+__device__ void lartg(cd f, cd g, double *c, cd *s, cd *r) {
+    const double fa = hypot(f.r, f.i), ga = hypot(g.r, g.i);
+    if (ga == 0.0) {
+        *c = 1.0, *s = cd{0, 0}, *r = f;
+    } else if (fa == 0.0) {
+        *c = 0.0, *s = cd{g.r / ga, -g.i / ga}, *r = cd{ga, 0.0};
+    } else {
+        const double d = hypot(fa, ga);
+        const cd ph = {f.r / fa, f.i / fa};
+        *c = fa / d;
+        *s = cd{(ph.r * g.r + ph.i * g.i) / d, (ph.i * g.r - ph.r * g.i) / d};
+        *r = cd{ph.r * d, ph.i * d};
+    }
+}
+
+// After CGS2 of column `col`: h[col][0..col] = h1 + h2, h[col][col+1] =
+// ||w||, breakdown test, past and new Givens rotations, S update, presid.
+// inv[s] receives 1/||w|| (0 on breakdown) for the basis store.
+__global__ void arnoldi_finish_kernel(int nsys, int restart, cd *H, cd *G,
+                                      cd *S, const cd *h1st, const cd *h2nd,
+                                      const cd *nrm_w, SysState *st,
+                                      double *inv, const int *active,
+                                      double eps) {
+    const int s = blockIdx.x * 64 + threadIdx.x;
+    if (s >= nsys || !active[s]) return;
+    SysState &q = st[s];
+    const int col = q.col;
+    cd *h = H + (size_t)s * restart * (restart + 1) + (size_t)col * (restart + 1);
+    cd *g = G + (size_t)s * restart * 2;
+    cd *Sv = S + (size_t)s * (restart + 1);
+    for (int r = 0; r <= col; ++r) {
+        const cd a = h1st[(size_t)s * (restart + 1) + r];
+        const cd b = h2nd[(size_t)s * (restart + 1) + r];
+        h[r] = cd{a.r + b.r, a.i + b.i};
+    }
+    const double h1 = sqrt(nrm_w[s].r);
+    h[col + 1] = cd{h1, 0.0};
+    int breakdown = 0;
+    if (h1 <= eps * q.h0) {
+        h[col + 1] = cd{0.0, 0.0};
+        breakdown = 1;
+        inv[s] = 0.0;
+    } else {
+        inv[s] = 1.0 / h1;
+    }
+    for (int r = 0; r < col; ++r) {
+        const double c = g[2 * r].r;
+        const cd sn = g[2 * r + 1];
+        const cd n0 = h[r], n1 = h[r + 1];
+        h[r] = cd{c * n0.r + (sn.r * n1.r - sn.i * n1.i),
+                  c * n0.i + (sn.r * n1.i + sn.i * n1.r)};
+        // -conj(s) n0 + c n1
+        h[r + 1] = cd{-(sn.r * n0.r + sn.i * n0.i) + c * n1.r,
+                      -(sn.r * n0.i - sn.i * n0.r) + c * n1.i};
+    }
+    double c;
+    cd sn, mag;
+    lartg(h[col], h[col + 1], &c, &sn, &mag);
+    g[2 * col] = cd{c, 0.0};
+    g[2 * col + 1] = sn;
+    h[col] = mag;
+    h[col + 1] = cd{0.0, 0.0};
+    const cd Sc = Sv[col];
+    const cd tmp = {-(sn.r * Sc.r + sn.i * Sc.i), -(sn.r * Sc.i - sn.i * Sc.r)};
+    Sv[col] = cd{c * Sc.r, c * Sc.i};
+    Sv[col + 1] = tmp;
+    q.presid = hypot(tmp.r, tmp.i);
+    q.breakdown = breakdown;
+    q.iters += 1;
+}
+
+// Upper-triangular solve of the Hessenberg least squares (SciPy's loop),
+// y stored per system; ncol = col + 1 columns.
+__global__ void lsq_kernel(int nsys, int restart, const cd *H, cd *S, cd *Y,
+                           const SysState *st, const int *active) {
+    const int s = blockIdx.x * 64 + threadIdx.x;
+    if (s >= nsys || !active[s]) return;
+    const int col = st[s].col;
+    const cd *h = H + (size_t)s * restart * (restart + 1);
+    cd *Sv = S + (size_t)s * (restart + 1);
+    cd *y = Y + (size_t)s * restart;
+    auto hh = [&](int c, int r) { return h[(size_t)c * (restart + 1) + r]; };
+    const cd dcc = hh(col, col);
+    if (dcc.r == 0.0 && dcc.i == 0.0) Sv[col] = cd{0.0, 0.0};
+    for (int r = 0; r <= col; ++r) y[r] = Sv[r];
+    for (int kk = col; kk > 0; --kk) {
+        if (y[kk].r != 0.0 || y[kk].i != 0.0) {
+            const cd d = hh(kk, kk);
+            const double dn = d.r * d.r + d.i * d.i;
+            y[kk] = cd{(y[kk].r * d.r + y[kk].i * d.i) / dn,
+                       (y[kk].i * d.r - y[kk].r * d.i) / dn};
+            const cd t = y[kk];
+            for (int r = 0; r < kk; ++r) {
+                const cd a = hh(kk, r);
+                y[r].r -= t.r * a.r - t.i * a.i;
+                y[r].i -= t.r * a.i + t.i * a.r;
+            }
+        }
+    }
+    if (y[0].r != 0.0 || y[0].i != 0.0) {
+        const cd d = hh(0, 0);
+        const double dn = d.r * d.r + d.i * d.i;
+        y[0] = cd{(y[0].r * d.r + y[0].i * d.i) / dn,
+                  (y[0].i * d.r - y[0].r * d.i) / dn};
+    }
+    for (int r = col + 1; r < restart; ++r) y[r] = cd{0.0, 0.0};
+}
+
+__global__ void exit_wave_kernel(const cd *__restrict__ v,
+                                 const cd *__restrict__ E, int nyc, int Nx,
+                                 size_t n, cd *__restrict__ out) {
+    for (size_t t = (size_t)blockIdx.x * 256 + threadIdx.x; t < n;
+         t += (size_t)gridDim.x * 256) {
+        const size_t pix = t % ((size_t)nyc * Nx);
+        out[t] = cconjmul(E[pix], v[t]);
+    }
+}
+
+// ---- workspace layout --------------------------------------------------
+struct Work {
+    cd *V, *b, *r, *w, *av, *H, *G, *S, *Y, *h1, *h2, *nrm, *pinv, *part;
+    double *inv;
+    SysState *st;
+    int *active;
+    size_t bytes;
+};
+
+static size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+static int reduce_parts(size_t N) {
+    const int P = (int)std::min<size_t>((N + 4095) / 4096, 64);
+    return std::max(P, 1);
+}
+
+static Work carve(void *base, int nsys, size_t N, int restart, int Nx) {
+    Work w{};
+    char *p = (char *)base;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        char *q = p ? p + off : nullptr;
+        off += align256(bytes);
+        return q;
+    };
+    const size_t R = restart;
+    const int P = reduce_parts(N);
+    w.V = (cd *)take(sizeof(cd) * nsys * (R + 1) * N);
+    w.b = (cd *)take(sizeof(cd) * nsys * N);
+    w.r = (cd *)take(sizeof(cd) * nsys * N);
+    w.w = (cd *)take(sizeof(cd) * nsys * N);
+    w.av = (cd *)take(sizeof(cd) * nsys * N);
+    w.H = (cd *)take(sizeof(cd) * nsys * R * (R + 1));
+    w.G = (cd *)take(sizeof(cd) * nsys * R * 2);
+    w.S = (cd *)take(sizeof(cd) * nsys * (R + 1));
+    w.Y = (cd *)take(sizeof(cd) * nsys * R);
+    w.h1 = (cd *)take(sizeof(cd) * nsys * (R + 1));
+    w.h2 = (cd *)take(sizeof(cd) * nsys * (R + 1));
+    w.nrm = (cd *)take(sizeof(cd) * nsys);
+    w.pinv = (cd *)take(sizeof(cd) * nsys * Nx);
+    w.part = (cd *)take(sizeof(cd) * nsys * (R + 2) * P);
+    w.inv = (double *)take(sizeof(double) * nsys);
+    w.st = (SysState *)take(sizeof(SysState) * nsys);
+    w.active = (int *)take(sizeof(int) * nsys);
+    w.bytes = off;
+    return w;
+}
+
+// dot products of kc vectors A_q (per system) against B, summed into out
+static int dots(const cd *A, size_t a_sys, size_t a_vec, int kc, const cd *Bv,
+                size_t b_sys, size_t N, cd *out, int ostride, cd *part,
+                int nsys, const int *active, hipStream_t stream) {
+    const int P = reduce_parts(N);
+    batched_dot_kernel<<<dim3(nsys * kc, P), kRedThreads, 0, stream>>>(
+        A, a_sys, a_vec, kc, Bv, b_sys, N, P, part, active);
+    DYF_LAUNCHED("batched_dot_kernel");
+    const int n = nsys * kc;
+    sum_parts_kernel<<<(n + 255) / 256, 256, 0, stream>>>(part, P, n, out,
+                                                          ostride, active, kc);
+    DYF_LAUNCHED("sum_parts_kernel");
+    return DYF_OK;
+}
+
+}  // namespace dyf
+
+using namespace dyf;
+
+static int check_bragg(const dyf_bragg_args *a, int nsys) {
+    if (!a || a->Nz < 2 || a->Nx < 2 || a->nyc < 1 || nsys < 1)
+        return fail(DYF_EINVAL, "bad Bragg arguments");
+    if (a->Nx > kMarchThreads * kMaxPerThread)
+        return fail(DYF_EINVAL, "Nx=%d exceeds %d", a->Nx,
+                    kMarchThreads * kMaxPerThread);
+    if (!(a->ds > 0.0)) return fail(DYF_EINVAL, "ds must be positive");
+    return DYF_OK;
+}
+
+extern "C" int dyf_bragg_march(const dyf_bragg_args *a, const double *s_dev,
+                               int nsys, const void *E, const void *incident,
+                               const void *Dg0, void *out, void *stream) {
+    int rc = check_bragg(a, nsys);
+    if (rc) return rc;
+    if (!s_dev || !E || !out) return fail(DYF_EINVAL, "NULL s_dev, E or out");
+    const BraggK k = make_k(a, nsys);
+    const size_t N = (size_t)a->nyc * a->Nx;
+    return launch_march(k, s_dev, (const cd *)E, (const cd *)incident,
+                        (const cd *)Dg0, N, (cd *)out, N, 1.0, nullptr,
+                        (hipStream_t)stream);
+}
+
+extern "C" int dyf_bragg_precondition(const dyf_bragg_args *a,
+                                      const double *s_dev, int nsys,
+                                      const void *v, void *w, void *pinv_ws,
+                                      void *stream_) {
+    int rc = check_bragg(a, nsys);
+    if (rc) return rc;
+    hipStream_t stream = (hipStream_t)stream_;
+    const BraggK k = make_k(a, nsys);
+    bragg_symbol_kernel<<<nsys, 256, 0, stream>>>(k, s_dev, (cd *)pinv_ws);
+    DYF_LAUNCHED("bragg_symbol_kernel");
+    return apply_precon(k, (const cd *)pinv_ws, (cd *)v, (cd *)w, stream);
+}
+
+extern "C" int dyf_bragg_workspace_bytes(const dyf_bragg_args *a, int nsys,
+                                         int restart, size_t *bytes) {
+    int rc = check_bragg(a, nsys);
+    if (rc) return rc;
+    if (restart < 1 || !bytes) return fail(DYF_EINVAL, "bad restart/bytes");
+    const size_t N = (size_t)a->nyc * a->Nx;
+    *bytes = carve(nullptr, nsys, N, restart, a->Nx).bytes;
+    return DYF_OK;
+}
+
+extern "C" int dyf_bragg_exit_wave(const dyf_bragg_args *a, int nsys,
+                                   const void *E, const void *v, void *out,
+                                   void *stream) {
+    int rc = check_bragg(a, nsys);
+    if (rc) return rc;
+    const size_t n = (size_t)nsys * a->nyc * a->Nx;
+    const int blocks = (int)std::min<size_t>((n + 255) / 256, 8192);
+    exit_wave_kernel<<<blocks, 256, 0, (hipStream_t)stream>>>(
+        (const cd *)v, (const cd *)E, a->nyc, a->Nx, n, (cd *)out);
+    DYF_LAUNCHED("exit_wave_kernel");
+    return DYF_OK;
+}
+
+extern "C" int dyf_bragg_solve(const dyf_bragg_args *a, const double *s_dev,
+                               int nsys, const void *E_, const void *incident,
+                               const dyf_gmres_opts *opt, const void *x0,
+                               void *x_, void *work, size_t work_bytes,
+                               dyf_gmres_info *info, void *stream_) {
+    int rc = check_bragg(a, nsys);
+    if (rc) return rc;
+    if (!opt || opt->restart < 1 || opt->maxiter < 1 || !(opt->tol > 0.0))
+        return fail(DYF_EINVAL, "bad GMRES options");
+    if (!s_dev || !E_ || !incident || !x_ || !work || !info)
+        return fail(DYF_EINVAL, "NULL argument to dyf_bragg_solve");
+    hipStream_t stream = (hipStream_t)stream_;
+    const BraggK k = make_k(a, nsys);
+    const size_t N = (size_t)a->nyc * a->Nx;
+    const int R = std::min<size_t>((size_t)opt->restart, N);
+    Work W = carve(work, nsys, N, R, a->Nx);
+    if (W.bytes > work_bytes)
+        return fail(DYF_EINVAL, "workspace of %zu bytes, need %zu", work_bytes,
+                    W.bytes);
+    const cd *E = (const cd *)E_;
+    cd *x = (cd *)x_;
+    const double eps = 2.220446049250313e-16;
+    const size_t Vsys = (size_t)(R + 1) * N;
+    const dim3 vgrid((unsigned)((N + 255) / 256), nsys);
+
+    std::vector<SysState> st(nsys);
+    std::vector<int> act(nsys, 1);
+    // Small host<->device state copies are ordered on the stream and
+    // completed before returning, so the host vectors can change freely.
+    auto push_state = [&]() -> int {
+        DYF_HIP(hipMemcpyAsync(W.st, st.data(), sizeof(SysState) * nsys,
+                               hipMemcpyHostToDevice, stream));
+        DYF_HIP(hipMemcpyAsync(W.active, act.data(), sizeof(int) * nsys,
+                               hipMemcpyHostToDevice, stream));
+        DYF_HIP(hipStreamSynchronize(stream));
+        return DYF_OK;
+    };
+    auto pull_state = [&]() -> int {
+        DYF_HIP(hipMemcpyAsync(st.data(), W.st, sizeof(SysState) * nsys,
+                               hipMemcpyDeviceToHost, stream));
+        DYF_HIP(hipStreamSynchronize(stream));
+        return DYF_OK;
+    };
+    std::vector<cd> nrm(nsys);
+    auto norms = [&](const cd *v, size_t sys_stride, const int *active) -> int {
+        int r2 = dots(v, sys_stride, 0, 1, v, sys_stride, N, W.nrm, 1, W.part,
+                      nsys, active, stream);
+        if (r2) return r2;
+        DYF_HIP(hipMemcpyAsync(nrm.data(), W.nrm, sizeof(cd) * nsys,
+                               hipMemcpyDeviceToHost, stream));
+        DYF_HIP(hipStreamSynchronize(stream));
+        return DYF_OK;
+    };
+
+    // preconditioner symbol for every system; b = -march(incident, 0)
+    bragg_symbol_kernel<<<nsys, 256, 0, stream>>>(k, s_dev, W.pinv);
+    DYF_LAUNCHED("bragg_symbol_kernel");
+    rc = launch_march(k, s_dev, E, (const cd *)incident, nullptr, 0, W.b, N,
+                      -1.0, nullptr, stream);
+    if (rc) return rc;
+    if (x0) {
+        DYF_HIP(hipMemcpyAsync(x, x0, sizeof(cd) * nsys * N,
+                               hipMemcpyDeviceToDevice, stream));
+    } else {
+        DYF_HIP(hipMemsetAsync(x, 0, sizeof(cd) * nsys * N, stream));
+    }
+    DYF_HIP(hipMemsetAsync(W.H, 0, sizeof(cd) * nsys * R * (R + 1), stream));
+    // ||b|| and ||M b||
+    rc = norms(W.b, N, nullptr);
+    if (rc) return rc;
+    for (int s = 0; s < nsys; ++s) {
+        st[s] = SysState{};
+        st[s].bnrm2 = sqrt(nrm[s].r);
+        st[s].atol = opt->tol * st[s].bnrm2;
+        st[s].pmf = 1.0;
+        st[s].code = opt->maxiter;
+        if (st[s].bnrm2 == 0.0) {   // SciPy returns b (= 0) at once
+            st[s].done = 1, st[s].code = 0, act[s] = 0;
+        }
+    }
+    rc = apply_precon(k, W.pinv, W.b, W.w, stream);
+    if (rc) return rc;
+    rc = norms(W.w, N, nullptr);
+    if (rc) return rc;
+    for (int s = 0; s < nsys; ++s) {
+        const double Mb = sqrt(nrm[s].r);
+        st[s].ptol = st[s].bnrm2 > 0.0
+                         ? Mb * std::min(1.0, st[s].atol / st[s].bnrm2)
+                         : 0.0;
+    }
+    // r = b - A x (or b when x = 0)
+    if (x0) {
+        rc = launch_march(k, s_dev, E, nullptr, x, N, W.av, N, 1.0, nullptr,
+                          stream);
+        if (rc) return rc;
+        sub_kernel<<<vgrid, 256, 0, stream>>>(W.b, W.av, W.r, N, nullptr);
+        DYF_LAUNCHED("sub_kernel");
+    } else {
+        DYF_HIP(hipMemcpyAsync(W.r, W.b, sizeof(cd) * nsys * N,
+                               hipMemcpyDeviceToDevice, stream));
+    }
+    rc = norms(W.r, N, nullptr);
+    if (rc) return rc;
+    for (int s = 0; s < nsys; ++s) {
+        st[s].rnorm = sqrt(nrm[s].r);
+        if (!st[s].done && st[s].rnorm < st[s].atol) {
+            st[s].done = 1, st[s].code = 0, act[s] = 0;
+        }
+    }
+
+    for (int cycle = 0; cycle < opt->maxiter; ++cycle) {
+        int any = 0;
+        for (int s = 0; s < nsys; ++s) {
+            act[s] = !st[s].done;
+            st[s].col = 0, st[s].breakdown = 0, st[s].inner_done = 0;
+            any |= act[s];
+        }
+        if (!any) break;
+        rc = push_state();
+        if (rc) return rc;
+        // v0 = M r / ||M r||, S = [||M r||, 0, ...]
+        rc = apply_precon(k, W.pinv, W.r, W.w, stream);
+        if (rc) return rc;
+        rc = norms(W.w, N, W.active);
+        if (rc) return rc;
+        std::vector<double> inv(nsys, 0.0);
+        std::vector<cd> S0((size_t)nsys * (R + 1), cd{0.0, 0.0});
+        for (int s = 0; s < nsys; ++s) {
+            const double t = sqrt(nrm[s].r);
+            inv[s] = t > 0.0 ? 1.0 / t : 0.0;
+            S0[(size_t)s * (R + 1)] = cd{t, 0.0};
+        }
+        DYF_HIP(hipMemcpyAsync(W.inv, inv.data(), sizeof(double) * nsys,
+                               hipMemcpyHostToDevice, stream));
+        DYF_HIP(hipMemcpyAsync(W.S, S0.data(), sizeof(cd) * S0.size(),
+                               hipMemcpyHostToDevice, stream));
+        DYF_HIP(hipStreamSynchronize(stream));
+        scale_copy_kernel<<<vgrid, 256, 0, stream>>>(W.w, N, W.V, Vsys, W.inv,
+                                                     N, W.active);
+        DYF_LAUNCHED("scale_copy_kernel");
+
+        for (int col = 0; col < R; ++col) {
+            // w = M A v_col
+            rc = launch_march(k, s_dev, E, nullptr, W.V + (size_t)col * N,
+                              Vsys, W.av, N, 1.0, W.active, stream);
+            if (rc) return rc;
+            rc = apply_precon(k, W.pinv, W.av, W.w, stream);
+            if (rc) return rc;
+            // h0 = ||w||; CGS2 against v_0..v_col
+            rc = dots(W.w, N, 0, 1, W.w, N, N, W.nrm, 1, W.part, nsys,
+                      W.active, stream);
+            if (rc) return rc;
+            DYF_HIP(hipMemcpyAsync(nrm.data(), W.nrm, sizeof(cd) * nsys,
+                                   hipMemcpyDeviceToHost, stream));
+            for (int pass = 0; pass < 2; ++pass) {
+                cd *hp = pass == 0 ? W.h1 : W.h2;
+                rc = dots(W.V, Vsys, N, col + 1, W.w, N, N, hp, R + 1, W.part,
+                          nsys, W.active, stream);
+                if (rc) return rc;
+                combine_kernel<<<vgrid, 256, 0, stream>>>(
+                    W.V, Vsys, N, col + 1, nullptr, hp, R + 1, -1.0, W.w, N, N,
+                    W.active);
+                DYF_LAUNCHED("combine_kernel");
+            }
+            DYF_HIP(hipStreamSynchronize(stream));
+            for (int s = 0; s < nsys; ++s)
+                if (act[s]) st[s].h0 = sqrt(nrm[s].r), st[s].col = col;
+            rc = push_state();
+            if (rc) return rc;
+            rc = dots(W.w, N, 0, 1, W.w, N, N, W.nrm, 1, W.part, nsys,
+                      W.active, stream);
+            if (rc) return rc;
+            arnoldi_finish_kernel<<<(nsys + 63) / 64, 64, 0, stream>>>(
+                nsys, R, W.H, W.G, W.S, W.h1, W.h2, W.nrm, W.st, W.inv,
+                W.active, eps);
+            DYF_LAUNCHED("arnoldi_finish_kernel");
+            if (col + 1 <= R) {
+                scale_copy_kernel<<<vgrid, 256, 0, stream>>>(
+                    W.w, N, W.V + (size_t)(col + 1) * N, Vsys, W.inv, N,
+                    W.active);
+                DYF_LAUNCHED("scale_copy_kernel");
+            }
+            rc = pull_state();
+            if (rc) return rc;
+            int still = 0;
+            for (int s = 0; s < nsys; ++s) {
+                if (!act[s]) continue;
+                if (st[s].presid <= st[s].ptol || st[s].breakdown ||
+                    col == R - 1) {
+                    st[s].inner_done = 1;
+                    act[s] = 0;
+                } else {
+                    still = 1;
+                }
+            }
+            rc = push_state();
+            if (rc) return rc;
+            if (!still) break;
+        }
+        // solve the small least-squares problems and update x
+        for (int s = 0; s < nsys; ++s) act[s] = !st[s].done;
+        rc = push_state();
+        if (rc) return rc;
+        lsq_kernel<<<(nsys + 63) / 64, 64, 0, stream>>>(nsys, R, W.H, W.S,
+                                                        W.Y, W.st, W.active);
+        DYF_LAUNCHED("lsq_kernel");
+        combine_kernel<<<vgrid, 256, 0, stream>>>(W.V, Vsys, N, R, W.st, W.Y,
+                                                  R, 1.0, x, N, N, W.active);
+        DYF_LAUNCHED("combine_kernel");
+        rc = launch_march(k, s_dev, E, nullptr, x, N, W.av, N, 1.0, W.active,
+                          stream);
+        if (rc) return rc;
+        sub_kernel<<<vgrid, 256, 0, stream>>>(W.b, W.av, W.r, N, W.active);
+        DYF_LAUNCHED("sub_kernel");
+        rc = norms(W.r, N, W.active);
+        if (rc) return rc;
+        for (int s = 0; s < nsys; ++s) {
+            if (st[s].done) continue;
+            SysState &q = st[s];
+            q.rnorm = sqrt(nrm[s].r);
+            if (q.rnorm <= q.atol) {
+                q.done = 1, q.code = 0;
+            } else if (q.breakdown) {
+                q.done = 1;
+            } else {
+                q.pmf = q.presid <= q.ptol ? std::max(eps, 0.25 * q.pmf)
+                                           : std::min(1.0, 1.5 * q.pmf);
+                q.ptol = q.presid * std::min(q.pmf, q.atol / q.rnorm);
+            }
+        }
+        // reset the Hessenberg for the next cycle
+        DYF_HIP(hipMemsetAsync(W.H, 0, sizeof(cd) * nsys * R * (R + 1), stream));
+    }
+    for (int s = 0; s < nsys; ++s) {
+        info[s].residual = st[s].bnrm2 > 0.0 ? st[s].rnorm / st[s].bnrm2 : 0.0;
+        info[s].iterations = st[s].iters;
+        info[s].code = st[s].rnorm <= st[s].atol ? 0 : opt->maxiter;
+    }
+    DYF_HIP(hipStreamSynchronize(stream));
+    return DYF_OK;
+}

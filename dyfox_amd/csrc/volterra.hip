@@ -135,9 +135,43 @@ __global__ __launch_bounds__(256) void phase_from_h_kernel(
     E[((size_t)r * nsteps + k) * (size_t)(C * kWave) + t] = e;
 }
 
+// ---- e^{iH} on Bragg chunk planes, (Nz, ny, Nx) ----------------------
+__global__ __launch_bounds__(256) void bragg_phase_kernel(
+    cx<double> *__restrict__ E, int nseg, int Nx, int ny, double dx,
+    double dz, double a2) {
+    const int ix = blockIdx.x * 256 + threadIdx.x;
+    const int k = blockIdx.y, j = blockIdx.z;
+    if (ix >= Nx) return;
+    const double x = ((double)ix - (double)Nx / 2.0) * dx;
+    const double H = volterra_h(c_segs, nseg, x, c_yvals[j], (double)k * dz, a2);
+    E[((size_t)k * ny + j) * Nx + ix] = phase_f64(H);
+}
+
 }  // namespace dyf
 
 using namespace dyf;
+
+extern "C" int dyf_bragg_phase(const double *segs, int nseg, const double *g,
+                               double nu, double a_core, const dyf_volume *vol,
+                               const double *y_values, int ny, void *E,
+                               void *stream_) {
+    hipStream_t stream = (hipStream_t)stream_;
+    if (!vol || vol->Nx <= 0 || vol->Nz <= 0 || vol->Nz > 65535)
+        return fail(DYF_EINVAL, "bad volume");
+    if (ny < 1 || ny > kMaxRows || !y_values || !E)
+        return fail(DYF_EINVAL, "bad planes (ny=%d, limit %d)", ny, kMaxRows);
+    int kept = 0;
+    int rc = upload_segments(segs, nseg, g, nu, &kept, stream);
+    if (rc) return rc;
+    DYF_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_yvals), y_values,
+                              sizeof(double) * ny, 0, hipMemcpyHostToDevice));
+    dim3 grid((vol->Nx + 255) / 256, vol->Nz, ny);
+    bragg_phase_kernel<<<grid, 256, 0, stream>>>(
+        (cx<double> *)E, kept, vol->Nx, ny, vol->dx, vol->dz,
+        a_core * a_core);
+    DYF_LAUNCHED("bragg_phase_kernel");
+    return DYF_OK;
+}
 
 extern "C" int dyf_volterra_h(const double *segs, int nseg, const double *g,
                               double nu, double a_core, const dyf_volume *vol,

@@ -142,6 +142,60 @@ int dyf_laue_march(const dyf_laue_args *a, int mode, int prec,
                    const dyf_line_params *line, const double *s_dev_dev,
                    int n_e, void *coef_dev, void *out_dev, void *stream);
 
+/* ---- Bragg shooting solve (bragg/solver.py:28-60, 135-379) ---------- */
+typedef struct {
+    double s0r, s0i, shr, shi, sbr, sbi;  /* sigma_0, sigma_h, sigma_hbar */
+    double ds;                            /* path step dz / sin(theta_B)  */
+    int32_t Nz, Nx, nyc, reserved;        /* depth planes, columns, y-planes */
+} dyf_bragg_args;
+
+typedef struct {
+    double tol;        /* SciPy rtol: stop when ||b - A x|| <= tol ||b||   */
+    int32_t restart;   /* Krylov dimension of one cycle                    */
+    int32_t maxiter;   /* restart cycles                                   */
+} dyf_gmres_opts;
+
+typedef struct {
+    double residual;   /* final ||b - A x|| / ||b|| (true residual)         */
+    int32_t iterations;/* inner iterations (SciPy's pr_norm callbacks)      */
+    int32_t code;      /* 0 converged, maxiter otherwise (SciPy's info)     */
+} dyf_gmres_info;
+
+/* e^{iH} on the (Nz, ny, Nx) chunk planes y_values[0..ny), x = (ix - Nx/2)
+ * dx, z = k dz (the stacked H_plane of bragg/solver.py:256-261).          */
+int dyf_bragg_phase(const double *segs, int nseg, const double *g_vec,
+                    double nu, double a_core, const dyf_volume *vol,
+                    const double *y_values, int ny, void *E_dev,
+                    void *stream);
+/* Dg(t) of the down-march for nsys systems over the chunk: D0(0) =
+ * incident_dev (Nx complex, broadcast; NULL = 0), Dg(0) = Dg0_dev
+ * (nsys, nyc, Nx; NULL = 0).  One system per s_dev_dev[s]
+ * (_march_down, bragg/solver.py:28-35).                                   */
+int dyf_bragg_march(const dyf_bragg_args *a, const double *s_dev_dev,
+                     int nsys, const void *E_dev, const void *incident_dev,
+                     const void *Dg0_dev, void *out_dev, void *stream);
+/* w = M v: ifft(fft(v)/T22s) per plane with the floored perfect-crystal
+ * symbol (perfect_symbol + precon, bragg/solver.py:38-60, 163-168,
+ * 272-275); pinv_dev receives 1/(T22s Nx), nsys x Nx complex.            */
+int dyf_bragg_precondition(const dyf_bragg_args *a, const double *s_dev_dev,
+                           int nsys, const void *v_dev, void *w_dev,
+                           void *pinv_dev, void *stream);
+int dyf_bragg_workspace_bytes(const dyf_bragg_args *a, int nsys, int restart,
+                              size_t *bytes);
+/* Restarted, left-preconditioned GMRES on A v = -d for nsys systems at
+ * once (one per s_dev), A v = march(0, v), d = march(incident, 0): the
+ * scipy.sparse.linalg.gmres call of bragg/solver.py:330-334 with the
+ * Arnoldi basis, orthogonalisation and Givens rotations on the device.
+ * x0_dev (nsys, nyc, Nx) warm start or NULL; x_dev receives v.            */
+int dyf_bragg_solve(const dyf_bragg_args *a, const double *s_dev_dev,
+                    int nsys, const void *E_dev, const void *incident_dev,
+                    const dyf_gmres_opts *opts, const void *x0_dev,
+                    void *x_dev, void *work_dev, size_t work_bytes,
+                    dyf_gmres_info *info, void *stream);
+/* Exit wave v e^{-iH(z=0)} (bragg/solver.py:365).                        */
+int dyf_bragg_exit_wave(const dyf_bragg_args *a, int nsys, const void *E_dev,
+                        const void *v_dev, void *out_dev, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,181 @@
+"""Bragg shooting solve on the device against the oracle.
+
+Bars: the march, the preconditioner and e^{iH} agree with the NumPy
+transcriptions to 1e-12 relative; GMRES solutions (true residual <= tol
+on both sides) agree with the oracle's to 1e-8 relative, with a dense
+brute-force solve on a tiny grid, and with the discrete perfect-crystal
+closed form; batching and chunking leave the answer unchanged.
+"""
+
+import numpy as np
+import pytest
+
+from dyfox_amd.bragg import (BraggConfig, bragg_g_vector, solve_bragg_stable,
+                             solve_surface_batch, threading_segment)
+from dyfox_amd.bragg import solver as BS
+from dyfox_amd.units import deviation_from_urad
+from oracle import bragg as OB, closed_forms as CF
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def g(diamond400):
+    return bragg_g_vector(diamond400)
+
+
+def _rel(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max()
+                 / np.abs(np.asarray(b)).max())
+
+
+def test_phase_march_and_preconditioner_match_oracle(dev, diamond400, g, rng):
+    cfg = BraggConfig(diamond400, t_crystal=6e-6, dx=0.2e-6, half_x=2e-6)
+    seg = threading_segment("Edge", +1, diamond400.b_mag, float(g[2]))
+    ys = np.array([-0.3e-6, 0.1e-6, 0.5e-6])
+    E = BS.chunk_phase(seg, cfg, diamond400, g, ys)
+    Ch, Cb, _ = OB.couplings(seg, cfg, diamond400, g, ys)
+    Eo = Cb / complex(diamond400.sig_hbar)
+    assert _rel(E.get(), Eo) < 1e-12
+    s = deviation_from_urad(np.array([-6.5, 12.0]), diamond400)
+    v = rng.normal(size=(2, 3, cfg.Nx)) + 1j * rng.normal(size=(2, 3, cfg.Nx))
+    inc = cfg.incident(n=1)[0]
+    for j, sd in enumerate(s):
+        e0, eg = OB.step_constants(sd, complex(diamond400.sig0), cfg.ds)
+        want_h = OB.march_down(np.zeros((3, cfg.Nx), complex), v[j], Ch, Cb,
+                               e0, eg, cfg.ds, cfg.Nz)
+        want_p = OB.march_down(np.broadcast_to(inc, (3, cfg.Nx)).copy(),
+                               np.zeros((3, cfg.Nx), complex), Ch, Cb, e0,
+                               eg, cfg.ds, cfg.Nz)
+        got_h = BS.march(cfg, diamond400, [sd], E, Dg0=v[j][None]).get()[0]
+        got_p = BS.march(cfg, diamond400, [sd], E, incident=inc).get()[0]
+        assert _rel(got_h, want_h) < 1e-12
+        assert _rel(got_p, want_p) < 1e-12
+        _, T22 = OB.perfect_symbol(e0, eg, cfg.ds, cfg.Nz, cfg.Nx,
+                                   complex(diamond400.sig_h),
+                                   complex(diamond400.sig_hbar))
+        T22s = OB.floored_symbol(T22)
+        want_w = np.fft.ifft(np.fft.fft(v[j], axis=-1) / T22s, axis=-1)
+        got_w = BS.precondition(cfg, diamond400, [sd], v[j][None])[0]
+        assert _rel(got_w, want_w) < 1e-12
+
+
+def test_surface_batch_matches_oracle(dev, diamond400, g):
+    cfg = BraggConfig(diamond400, t_crystal=6e-6, dx=0.1e-6, half_x=3e-6)
+    sd = float(deviation_from_urad(-6.5, diamond400))
+    seg = threading_segment("Edge", +1, diamond400.b_mag, float(g[2]))
+    ys = np.array([-0.2e-6, 0.0, 0.3e-6])
+    got, info = solve_surface_batch(seg, sd, cfg, ys, diamond400, g)
+    want, winfo = OB.solve_surface_batch(seg, sd, cfg, ys, diamond400, g)
+    print(info, winfo)
+    assert info["converged"] and info["residual"] < 1e-9
+    for key in ("residual", "iterations", "retries", "splits",
+                "unconverged_chunks", "converged"):
+        assert key in info
+    assert _rel(got, want) < 1e-8
+    for j, y in enumerate(ys):
+        one, oi = solve_bragg_stable(seg, sd, cfg, diamond400, g,
+                                     y_plane=float(y))
+        assert oi["residual"] < 1e-9 and oi["gmres_code"] == 0
+        assert np.abs(got[j] - one).max() < 1e-7 * np.abs(one).max()
+
+
+def test_dense_brute_force_on_a_tiny_grid(dev, diamond400, g):
+    tiny = BraggConfig(diamond400, 3e-6, 0.2e-6, 1e-6)
+    seg = threading_segment("Edge", +1, diamond400.b_mag, float(g[2]))
+    sd = float(deviation_from_urad(-6.5, diamond400))
+    dense, _, _ = OB.dense_shooting_solution(seg, sd, tiny, 0.1e-6,
+                                             diamond400, g)
+    got, info = solve_bragg_stable(seg, sd, tiny, diamond400, g, y_plane=0.1e-6)
+    assert info["residual"] < 1e-10
+    assert _rel(got, dense) < 1e-9
+
+
+def test_perfect_crystal_is_the_discrete_closed_form(dev, diamond400, g):
+    cfg = BraggConfig(diamond400, 20e-6, 0.5e-6, 32e-6)
+    s = deviation_from_urad(np.linspace(-60, 60, 9), diamond400)
+    out, infos = solve_surface_batch([], s, cfg, np.zeros(2), diamond400, g)
+    s0, sh, sb = (complex(diamond400.sig0), complex(diamond400.sig_h),
+                  complex(diamond400.sig_hbar))
+    for j, sd in enumerate(s):
+        want = CF.bragg_discrete_perfect(sd, s0, sh, sb, cfg.ds, cfg.Nz)
+        assert abs(out[j, 0, cfg.Nx // 2] - want) < 1e-11
+        assert infos[j]["converged"] and infos[j]["iterations"] <= 2
+
+
+def test_batched_angles_equal_one_at_a_time(dev, diamond400, g):
+    cfg = BraggConfig(diamond400, t_crystal=6e-6, dx=0.2e-6, half_x=2e-6)
+    seg = threading_segment("Edge", +1, diamond400.b_mag, float(g[2]),
+                            y0=-0.5e-6)
+    ys = (np.arange(4) - 2) * cfg.dx
+    s = deviation_from_urad(np.array([-20.0, -6.5, 0.0, 15.0]), diamond400)
+    out, infos = solve_surface_batch(seg, s, cfg, ys, diamond400, g)
+    for j, sd in enumerate(s):
+        one, info = solve_surface_batch(seg, float(sd), cfg, ys, diamond400, g)
+        assert infos[j]["converged"] and info["converged"]
+        assert _rel(out[j], one) < 1e-8
+
+
+def test_big_chunk_equals_small_chunk(dev, diamond400, g):
+    cfg = BraggConfig(diamond400, t_crystal=8e-6, dx=0.15e-6, half_x=3e-6)
+    sd = float(deviation_from_urad(-6.5, diamond400))
+    seg = threading_segment("Edge", +1, diamond400.b_mag, float(g[2]),
+                            y0=-1.5e-6)
+    ys = (np.arange(16) - 8) * cfg.dx
+    big, ib = solve_surface_batch(seg, sd, cfg, ys, diamond400, g, chunk=16)
+    one, i1 = solve_surface_batch(seg, sd, cfg, ys, diamond400, g, chunk=1)
+    assert ib["converged"] and i1["converged"]
+    assert ib["residual"] < 1e-7 and i1["residual"] < 1e-7
+    assert _rel(big, one) < 1e-6
+
+
+def test_retry_and_split_policy_is_followed(dev, diamond400, g):
+    """Force the policy from the host: an unattainable `accept` makes
+    every attempt retry until the cap, then split down to max_depth."""
+    cfg = BraggConfig(diamond400, t_crystal=6e-6, dx=0.2e-6, half_x=2e-6)
+    sd = float(deviation_from_urad(-6.5, diamond400))
+    seg = threading_segment("Edge", +1, diamond400.b_mag, float(g[2]))
+    ys = (np.arange(4) - 2) * cfg.dx
+    out, info = solve_surface_batch(seg, sd, cfg, ys, diamond400, g,
+                                    accept=1e-30, restart=4, restart_cap=16,
+                                    per_plane=1, max_depth=1, tries=3)
+    assert info["retries"] > 0 and info["splits"] == 1
+    assert info["unconverged_chunks"] == 2 and info["converged"] is False
+    want, winfo = OB.solve_surface_batch(seg, sd, cfg, ys, diamond400, g,
+                                         accept=1e-30, restart=4,
+                                         restart_cap=16, per_plane=1,
+                                         max_depth=1, tries=3)
+    assert (info["retries"], info["splits"]) == (winfo["retries"],
+                                                 winfo["splits"])
+    assert _rel(out, want) < 1e-8
+
+
+def test_warm_start_reaches_the_same_answer_in_fewer_iterations(dev,
+                                                                diamond400, g):
+    cfg = BraggConfig(diamond400, t_crystal=6e-6, dx=0.2e-6, half_x=2e-6)
+    sd = float(deviation_from_urad(-6.5, diamond400))
+    seg = threading_segment("Edge", +1, diamond400.b_mag, float(g[2]))
+    ys = (np.arange(4) - 2) * cfg.dx
+    cold, ic = solve_surface_batch(seg, sd, cfg, ys, diamond400, g)
+    E = BS.chunk_phase(seg, cfg, diamond400, g, ys)
+    v_exact = cold * E.get()[0]                    # undo the exit phase
+    noisy = v_exact * (1 + 1e-4)
+    warm, iw = solve_surface_batch(seg, sd, cfg, ys, diamond400, g,
+                                   warm_start=noisy)
+    assert iw["converged"] and iw["iterations"] < ic["iterations"]
+    assert _rel(warm, cold) < 1e-8
+
+
+def test_cfg1_case_b_matches_oracle(dev, diamond400, g):
+    """The full cfg1 chunk (Nz 90, Nx 520, 32 planes) with one [101] edge
+    line at mid-depth, two angles checked against the oracle."""
+    from dyfox_amd.workloads import bragg_surface
+    cfg, segs, ys, s, gv = bragg_surface(diamond400, "B")
+    pick = s[[3, 20]]
+    out, infos = solve_surface_batch(segs, pick, cfg, ys, diamond400, gv)
+    for j, sd in enumerate(pick):
+        want, winfo = OB.solve_surface_batch(segs, float(sd), cfg, ys,
+                                             diamond400, gv)
+        print(infos[j], winfo)
+        assert infos[j]["converged"] and infos[j]["residual"] <= 1e-10
+        assert _rel(out[j], want) < 1e-8

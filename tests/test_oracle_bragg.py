@@ -1,0 +1,140 @@
+"""Pins the Bragg oracle (oracle/bragg.py, oracle/closed_forms.py) and the
+host-side Bragg API (geometry, closed-form reflectivities).
+
+* known answers from the reference's tests/test_bragg.py: Darwin width
+  3.588 urad, |R| <= 1, plateau centred at -6.764 urad, Riccati -> the
+  semi-infinite root when thick, 20 um is not semi-infinite;
+* the exact finite-thickness Riccati solution against SciPy's DOP853
+  integration of the same ODE (the reference's own method);
+* the shooting discretisation converges to Riccati at first order in dx;
+* oracle GMRES against a dense brute-force solve (A assembled column by
+  column) and against the discrete perfect-crystal closed form;
+* zero Burgers vector equals the perfect crystal.
+"""
+
+import numpy as np
+import pytest
+
+from dyfox_amd.bragg import (BraggConfig, bragg_g_vector, darwin_width_urad,
+                             reflectivity_riccati, reflectivity_semi_infinite,
+                             threading_segment)
+from dyfox_amd.dislocations import g_dot_b
+from dyfox_amd.units import deviation_from_urad
+from oracle import bragg as OB, closed_forms as CF
+
+
+@pytest.fixture(scope="module")
+def g(diamond400):
+    return bragg_g_vector(diamond400)
+
+
+def _sig(x):
+    return complex(x.sig0), complex(x.sig_h), complex(x.sig_hbar)
+
+
+def test_geometry_rules(diamond400, g):
+    cfg = BraggConfig(diamond400, t_crystal=30e-6, dx=0.05e-6, half_x=12e-6)
+    assert cfg.dz == pytest.approx(cfg.dx * diamond400.tan_tB, rel=1e-15)
+    assert cfg.ds == pytest.approx(cfg.dz / diamond400.sin_tB, rel=1e-15)
+    assert cfg.Nx > 2 * (cfg.half_x / cfg.dx + 2 * cfg.Nz)
+    c1 = BraggConfig(diamond400, t_crystal=20e-6, dx=0.5e-6, half_x=32e-6)
+    assert (c1.Nz, c1.Nx) == (90, 520)
+    assert g[0] == 0.0 and g[1] == 0.0
+    for kind in ("Edge", "Screw"):
+        for sign in (+1, -1):
+            seg = threading_segment(kind, sign, diamond400.b_mag, float(g[2]))
+            assert g_dot_b(g, seg[0][2]) == pytest.approx(2.0 * sign, abs=1e-9)
+            assert g_dot_b(diamond400.g_vec, seg[0][2]) == pytest.approx(
+                0.0, abs=1e-9)
+
+
+def test_darwin_and_plateau(diamond400):
+    assert darwin_width_urad(diamond400) == pytest.approx(3.588, abs=0.01)
+    s0, sh, sb = _sig(diamond400)
+    assert CF.darwin_width_urad(sh, sb, diamond400.lam,
+                                diamond400.sin_2tB) == pytest.approx(
+        darwin_width_urad(diamond400), rel=1e-14)
+    s = deviation_from_urad(np.linspace(-30, 20, 401), diamond400)
+    assert np.abs(reflectivity_semi_infinite(s, diamond400)).max() <= 1 + 1e-12
+    th = np.linspace(-14, 4, 3601)
+    R = np.abs(reflectivity_semi_infinite(deviation_from_urad(th, diamond400),
+                                          diamond400))
+    top = th[R > 0.99 * R.max()]
+    assert top.max() - top.min() == pytest.approx(
+        darwin_width_urad(diamond400), rel=0.25)
+    assert 0.5 * (top.max() + top.min()) == pytest.approx(-6.764, abs=0.2)
+
+
+def test_riccati_thick_and_thin(diamond400):
+    s = deviation_from_urad(np.array([-6.7644]), diamond400)
+    bulk = np.abs(reflectivity_semi_infinite(s, diamond400))[0]
+    thin = np.abs(reflectivity_riccati(s, 5e-6, diamond400))[0]
+    thick = np.abs(reflectivity_riccati(s, 60e-6, diamond400))[0]
+    at20 = np.abs(reflectivity_riccati(s, 20e-6, diamond400))[0]
+    assert abs(thick - bulk) < abs(thin - bulk)
+    assert thick == pytest.approx(bulk, abs=0.02)
+    assert abs(at20 - bulk) > 1e-3
+
+
+def test_riccati_closed_form_matches_ode_integration(diamond400):
+    from scipy.integrate import solve_ivp
+    s0, sh, sb = _sig(diamond400)
+    t = 20e-6
+    for urad in (-60.0, -6.5, 0.0, 3.0, 60.0):
+        sd = float(deviation_from_urad(urad, diamond400))
+        al = -2 * np.pi * sd
+
+        def rhs(z, R):
+            Rc = R[0] + 1j * R[1]
+            d = -(sh + (2 * s0 + 1j * al) * Rc + sb * Rc * Rc) / diamond400.sin_tB
+            return [d.real, d.imag]
+
+        with np.errstate(all="ignore"):
+            sol = solve_ivp(rhs, [t, 0.0], [0.0, 0.0], rtol=1e-11, atol=1e-14,
+                            method="DOP853")
+        want = sol.y[0, -1] + 1j * sol.y[1, -1]
+        got = reflectivity_riccati([sd], t, diamond400)[0]
+        oracle = CF.reflectivity_finite([sd], t, s0, sh, sb,
+                                        diamond400.sin_tB)[0]
+        assert abs(got - want) < 1e-9 and abs(oracle - want) < 1e-9
+
+
+def test_shooting_converges_to_riccati_at_first_order(diamond400):
+    s0, sh, sb = _sig(diamond400)
+    s = deviation_from_urad(np.linspace(-60, 60, 41), diamond400)
+    errs = []
+    for dx in (0.5e-6, 0.1e-6, 0.05e-6):
+        cfg = BraggConfig(diamond400, 20e-6, dx, 2e-6)
+        d = np.array([CF.bragg_discrete_perfect(sd, s0, sh, sb, cfg.ds, cfg.Nz)
+                      for sd in s])
+        R = reflectivity_riccati(s, cfg.t_crystal, diamond400)
+        errs.append(np.abs(d - R).max())
+    assert errs[0] < 0.04
+    assert errs[1] / errs[2] == pytest.approx(2.0, rel=0.15)
+
+
+def test_oracle_gmres_matches_dense_and_closed_form(diamond400, g):
+    s0, sh, sb = _sig(diamond400)
+    sd = float(deviation_from_urad(-6.5, diamond400))
+    tiny = BraggConfig(diamond400, 3e-6, 0.2e-6, 1e-6)
+    seg = threading_segment("Edge", +1, diamond400.b_mag, float(g[2]))
+    dense, _, _ = OB.dense_shooting_solution(seg, sd, tiny, 0.1e-6,
+                                             diamond400, g)
+    out, info = OB.solve_surface_batch(seg, sd, tiny, np.array([0.1e-6]),
+                                       diamond400, g)
+    assert info["converged"]
+    assert np.abs(out[0] - dense).max() < 1e-9 * np.abs(dense).max()
+    cfg = BraggConfig(diamond400, 20e-6, 0.5e-6, 32e-6)
+    out, info = OB.solve_surface_batch([], sd, cfg, np.zeros(2), diamond400, g)
+    want = CF.bragg_discrete_perfect(sd, s0, sh, sb, cfg.ds, cfg.Nz)
+    assert abs(out[0, cfg.Nx // 2] - want) < 1e-12
+
+
+def test_oracle_zero_b_is_perfect(diamond400, g):
+    cfg = BraggConfig(diamond400, t_crystal=6e-6, dx=0.2e-6, half_x=2e-6)
+    sd = float(deviation_from_urad(-6.5, diamond400))
+    seg = threading_segment("Edge", +1, diamond400.b_mag, float(g[2]))
+    dead = [(seg[0][0], seg[0][1], np.zeros(3))]
+    a, _ = OB.solve_surface_batch(dead, sd, cfg, np.zeros(1), diamond400, g)
+    b, _ = OB.solve_surface_batch([], sd, cfg, np.zeros(1), diamond400, g)
+    assert np.abs(a - b).max() < 1e-12
