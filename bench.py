@@ -82,6 +82,27 @@ def time_laue(N, segs, s, grid, xtal, prec, steps, warmup):
     return wall, phase_ms, march_ms, out
 
 
+def time_bragg(xtal, case, repeats=3):
+    """cfg1: 41 s_dev over one 32-plane chunk, solved as one batched
+    GMRES per repeat (cold start); returns solves/s and per-solve info."""
+    from dyfox_amd.bragg import solve_surface_batch
+    from dyfox_amd.workloads import bragg_surface
+    cfg, segs, ys, s, g = bragg_surface(xtal, case)
+    solve_surface_batch(segs, s, cfg, ys, xtal, g)          # plans, warm-up
+    times = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        _, infos = solve_surface_batch(segs, s, cfg, ys, xtal, g)
+        times.append(time.perf_counter() - t0)
+    dt = float(np.median(times))
+    its = [i["iterations"] for i in infos]
+    return {"solves_per_s": len(s) / dt, "ms_per_batch": dt * 1e3,
+            "n_solves": len(s), "gmres_iterations": [min(its), max(its)],
+            "worst_residual": max(i["residual"] for i in infos),
+            "all_converged": all(i["converged"] for i in infos),
+            "grid": f"Nz={cfg.Nz} Nx={cfg.Nx} planes={len(ys)}"}
+
+
 def cpu_baseline(segs, s, grid, xtal, rows=48, n_s=8):
     """The NumPy FP64 oracle (Volterra H + direct-form march) on a bounded
     sample: `rows` marched rows x `n_s` angles of the same workload."""
@@ -186,6 +207,10 @@ def main():
     print(f"[bench] cfg2 fp64 step {w2 * 1e3:.2f} ms", file=sys.stderr,
           flush=True)
 
+    bragg = {c: time_bragg(xtal, c) for c in ("A", "B")}
+    print(f"[bench] bragg cfg1 B {bragg['B']['solves_per_s']:.1f} solves/s",
+          file=sys.stderr, flush=True)
+
     achieved = FLOPS_PER_PIXEL_STEP * work / (m64 * 1e-3) / 1e12
     roof = {"kernel": "laue_fast_kernel<C=17, GRID, FP64>",
             "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
@@ -219,6 +244,7 @@ def main():
         "worked_example_fp64": {"value": work2 / w2, "ms_per_step": w2 * 1e3,
                                 "phase_ms": p2, "march_ms": m2,
                                 "pixel_steps": work2},
+        "bragg_cfg1_cold": bragg,
         "roofline": roof,
     }
     if not args.no_cpu:
