@@ -47,6 +47,17 @@ class LineParams(C.Structure):
                  "ge2", "b_perp", "nu", "a2")]
 
 
+class LaueInstance(C.Structure):
+    _fields_ = [("grid", LaueArgs), ("segs", C.POINTER(C.c_double)),
+                ("nseg", C.c_int32), ("n_e", C.c_int32),
+                ("g_vec", C.c_double * 3), ("nu", C.c_double),
+                ("a_core", C.c_double), ("s_dev", vp), ("out", vp)]
+
+
+class MarchTune(C.Structure):
+    _fields_ = [("waves_per_block", C.c_int32), ("blocks_per_cu", C.c_int32)]
+
+
 class BraggArgs(C.Structure):
     _fields_ = [(n, C.c_double) for n in
                 ("s0r", "s0i", "shr", "shi", "sbr", "sbi", "ds")] + \
@@ -98,6 +109,18 @@ _SIGNATURES = {
                                 C.POINTER(LineParams), vp, vp],
     "dyf_laue_march": [C.POINTER(LaueArgs), C.c_int, C.c_int, vp, C.c_int,
                        C.POINTER(LineParams), vp, C.c_int, vp, vp, vp],
+    "dyf_laue_batch_bytes": [C.POINTER(LaueInstance), C.c_int, C.c_int,
+                             C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)],
+    "dyf_laue_batch_prepare": [C.POINTER(LaueInstance), C.c_int, C.c_int, vp,
+                               C.c_size_t, vp, C.c_size_t],
+    "dyf_laue_batch_run": [C.POINTER(LaueInstance), C.c_int, C.c_int, vp, vp,
+                           C.POINTER(MarchTune), C.c_int, vp],
+    "dyf_graph_begin": [vp],
+    "dyf_graph_end": [vp, C.POINTER(vp)],
+    "dyf_graph_launch": [vp, vp],
+    "dyf_graph_destroy": [vp],
+    "dyf_stream_create": [C.POINTER(vp)],
+    "dyf_stream_destroy": [vp],
     "dyf_bragg_phase": [dp, C.c_int, dp, C.c_double, C.c_double,
                         C.POINTER(VolumeDesc), dp, C.c_int, vp, vp],
     "dyf_bragg_march": [C.POINTER(BraggArgs), vp, C.c_int, vp, vp, vp, vp,

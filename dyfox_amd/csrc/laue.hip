@@ -25,8 +25,7 @@
 //
 // General path (any shift, any Nx): the reference's direct form, FP64, one
 // workgroup per (row, s_dev) with the rows double-buffered in LDS.
-#include "common.hpp"
-#include "phase_layout.hpp"
+#include "internal.hpp"
 
 namespace dyf {
 
@@ -49,7 +48,7 @@ struct CoefPtrs {
     CoefF *f;
     double *qN;
 };
-static CoefPtrs coef_ptrs(void *ws, int n_e, int nsteps) {
+__host__ __device__ inline CoefPtrs coef_ptrs(void *ws, int n_e, int nsteps) {
     char *p = (char *)ws;
     CoefPtrs c;
     c.d = (CoefD *)p;
@@ -184,15 +183,13 @@ __device__ __forceinline__ T fmaT(T a, T b, T c) {
         return __builtin_fmaf(a, b, c);
 }
 
+// March one (row r, angle ie) with the calling wavefront (64 lanes).
 template <int C, int MODE, int PREC>
-__global__ __launch_bounds__(64) void laue_fast_kernel(FastArgs a) {
+__device__ __forceinline__ void fast_row(const FastArgs &a, int r, int ie) {
     using S = StateT<PREC>;
     using T = typename S::T;
     constexpr bool COMP = S::comp;
-    const int lane = threadIdx.x;
-    const int L = logical_block(a.per_xcd);
-    if ((int)(blockIdx.x >> 3) >= a.per_xcd || L >= a.total) return;
-    const int r = L / a.n_e, ie = L - r * a.n_e;
+    const int lane = threadIdx.x & (kWave - 1);
     const int iy = a.iy_lo + r;
 
     // fields: hi parts (and compensations for MIXED)
@@ -323,6 +320,14 @@ __global__ __launch_bounds__(64) void laue_fast_kernel(FastArgs a) {
         else
             ((cx<float> *)a.out)[row + x] = cx<float>{(float)o_r, (float)o_i};
     }
+}
+
+template <int C, int MODE, int PREC>
+__global__ __launch_bounds__(64) void laue_fast_kernel(FastArgs a) {
+    const int L = logical_block(a.per_xcd);
+    if (L >= a.total) return;
+    const int r = L / a.n_e, ie = L - r * a.n_e;
+    fast_row<C, MODE, PREC>(a, r, ie);
 }
 
 // ---- general path: direct form, any shift, LDS rows ------------------
@@ -460,6 +465,112 @@ __global__ __launch_bounds__(256) void zero_rows_kernel(void *out, int elem,
         else
             ((cx<float> *)out)[at] = cx<float>{0.0f, 0.0f};
     }
+}
+
+// ---- persistent march over many instances ----------------------------
+__global__ void zero_counters_kernel(unsigned *c) {
+    if (threadIdx.x < 8) c[threadIdx.x] = 0u;
+}
+
+// One persistent kernel per column count C (register allocation is per
+// kernel, so each bucket keeps the occupancy of its standalone kernel).
+template <int C, int PREC>
+__global__ __launch_bounds__(256) void laue_persistent_kernel(
+    const InstDesc *__restrict__ inst, int n_inst, long long total,
+    long long per_queue, unsigned *__restrict__ counters) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int q0 = blockIdx.x & 7;   // block b runs on XCD b % 8
+    for (int qq = 0; qq < 8; ++qq) {
+        const int q = (q0 + qq) & 7;
+        const long long lo = (long long)q * per_queue;
+        const long long hi = lo + per_queue < total ? lo + per_queue : total;
+        for (;;) {
+            unsigned t = 0;
+            if (lane == 0) t = atomicAdd(&counters[q], 1u);
+            t = __builtin_amdgcn_readfirstlane(t);
+            const long long item = lo + (long long)t;
+            if (item >= hi) break;
+            int a = 0, b = n_inst - 1;
+            while (a < b) {
+                const int m = (a + b + 1) >> 1;
+                if (inst[m].item_base <= item) a = m; else b = m - 1;
+            }
+            const InstDesc &d = inst[a];
+            const long long local = item - d.item_base;
+            const int r = (int)(local / d.n_e), ie = (int)(local % d.n_e);
+            FastArgs fa;
+            fa.phase = d.phase;
+            const CoefPtrs cp = coef_ptrs((void *)d.coef, d.n_e, d.nsteps);
+            fa.cd = cp.d, fa.cf = cp.f, fa.qN = cp.qN;
+            fa.out = d.out;
+            fa.n_e = d.n_e, fa.nrows = d.nrows, fa.iy_lo = d.iy_lo;
+            fa.nsteps = d.nsteps, fa.Nx = d.Nx, fa.Ny = d.Ny;
+            fa.per_xcd = 0, fa.total = 0;
+            fa.dx = d.dx, fa.sheet_hw = d.sheet_hw;
+            fast_row<C, DYF_MODE_GRID, PREC>(fa, r, ie);
+        }
+    }
+}
+
+using PersistFn = void (*)(const InstDesc *, int, long long, long long,
+                           unsigned *);
+
+template <int PREC, int... Cs>
+static PersistFn persist_fn(int C) {
+    PersistFn f = nullptr;
+    ((C == Cs ? (f = laue_persistent_kernel<Cs, PREC>, 0) : 0), ...);
+    return f;
+}
+
+size_t laue_coef_bytes(int n_e, int nsteps) { return coef_bytes(n_e, nsteps); }
+
+int launch_laue_coef(const dyf_laue_args *a, const double *s_dev, int n_e,
+                     void *coef, hipStream_t stream) {
+    const int nsteps = a->Nz - 1;
+    const CoefPtrs cp = coef_ptrs(coef, n_e, nsteps);
+    const int n = n_e * nsteps;
+    laue_coef_kernel<<<(n + 255) / 256, 256, 0, stream>>>(
+        s_dev, n_e, nsteps, a->ds, a->s0r, a->s0i, a->shr, a->shi, a->sbr,
+        a->sbi, cp.d, cp.f, cp.qN);
+    DYF_LAUNCHED("laue_coef_kernel");
+    return DYF_OK;
+}
+
+int launch_zero_rows(const dyf_laue_args *a, const PhaseLayout &L, int prec,
+                     int n_e, void *out, hipStream_t stream) {
+    if (L.nrows >= a->Ny || n_e == 0) return DYF_OK;
+    const int elem = prec == DYF_FP64 ? 16 : 8;
+    const size_t n = (size_t)(a->Ny - L.nrows) * a->Nx * n_e;
+    const int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
+    zero_rows_kernel<<<blocks, 256, 0, stream>>>(out, elem, n_e, a->Ny, a->Nx,
+                                                  L.iy_lo, L.iy_hi);
+    DYF_LAUNCHED("zero_rows_kernel");
+    return DYF_OK;
+}
+
+int launch_laue_persistent(const InstDesc *inst, int n_inst, long long total,
+                           int prec, int C, unsigned *counters, int wpb,
+                           int blocks, hipStream_t stream) {
+    if (n_inst < 1 || total < 1) return DYF_OK;
+    if (wpb < 1 || wpb > 4 || blocks < 1)
+        return fail(DYF_EINVAL, "bad persistent launch shape (%d waves, %d "
+                    "blocks)", wpb, blocks);
+    PersistFn fn = nullptr;
+    if (prec == DYF_FP64)
+        fn = persist_fn<DYF_FP64, 2, 4, 6, 8, 10, 12, 13, 14, 16, 17, 18>(C);
+    else if (prec == DYF_FP32)
+        fn = persist_fn<DYF_FP32, 2, 4, 6, 8, 10, 12, 13, 14, 16, 17, 18>(C);
+    else
+        fn = persist_fn<DYF_MIXED, 2, 4, 6, 8, 10, 12, 13, 14, 16, 17, 18>(C);
+    if (!fn) return fail(DYF_EKEY, "no persistent kernel for C=%d", C);
+    zero_counters_kernel<<<1, 64, 0, stream>>>(counters);
+    DYF_LAUNCHED("zero_counters_kernel");
+    const long long per_queue = (total + 7) / 8;
+    if (per_queue > 0xffffffffLL) return fail(DYF_EINVAL, "too many items");
+    hipLaunchKernelGGL(fn, dim3(blocks), dim3(kWave * wpb), 0, stream, inst,
+                       n_inst, total, per_queue, counters);
+    DYF_LAUNCHED("laue_persistent_kernel");
+    return DYF_OK;
 }
 
 // ---- fast-kernel dispatch table ---------------------------------------

@@ -7,8 +7,7 @@
 // Segments live in constant memory (uniform across every wave, so every
 // read is a scalar load); calls on one device are serialised by stream
 // order, which is what makes one symbol per library enough.
-#include "common.hpp"
-#include "phase_layout.hpp"
+#include "internal.hpp"
 
 namespace dyf {
 
@@ -82,6 +81,50 @@ __global__ __launch_bounds__(256) void phase_from_segments_kernel(
             e = phase_f32(H);
     }
     E[((size_t)r * nsteps + k) * (size_t)(C * kWave) + t] = e;
+}
+
+// Same table with the segments read from device memory (uniform loads),
+// so a batch of instances can be enqueued - and graph-captured - without
+// touching the constant symbol.
+template <typename T>
+__global__ __launch_bounds__(256) void phase_segments_dev_kernel(
+    cx<T> *__restrict__ E, const SegParams *__restrict__ segs, int nseg,
+    int C, int Nx, int Ny, int nsteps, int iy_lo, double dx, double dz,
+    double a2) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int k = blockIdx.y;
+    const int r = blockIdx.z;
+    if (t >= C * kWave) return;
+    const int i = t / kWave, lane = t % kWave;
+    const int x = lane * C + i;
+    cx<T> e{T(0), T(0)};
+    if (x < Nx) {
+        const double xp = ((double)x - (double)Nx / 2.0) * dx;
+        const double yp = (double)(iy_lo + r - Ny / 2) * dx;
+        const double H = volterra_h(segs, nseg, xp, yp, (double)k * dz, a2);
+        if constexpr (sizeof(T) == 8)
+            e = phase_f64(H);
+        else
+            e = phase_f32(H);
+    }
+    E[((size_t)r * nsteps + k) * (size_t)(C * kWave) + t] = e;
+}
+
+int launch_phase_segments_dev(const dyf_laue_args *a, const PhaseLayout &L,
+                              int prec, const SegParams *segs, int nseg,
+                              double a2, void *phase, hipStream_t stream) {
+    if (L.nrows == 0) return DYF_OK;
+    dim3 grid((L.C * kWave + 255) / 256, L.nsteps, L.nrows);
+    if (prec == DYF_FP64)
+        phase_segments_dev_kernel<double><<<grid, 256, 0, stream>>>(
+            (cx<double> *)phase, segs, nseg, L.C, a->Nx, a->Ny, L.nsteps,
+            L.iy_lo, a->dx, a->dz, a2);
+    else
+        phase_segments_dev_kernel<float><<<grid, 256, 0, stream>>>(
+            (cx<float> *)phase, segs, nseg, L.C, a->Nx, a->Ny, L.nsteps,
+            L.iy_lo, a->dx, a->dz, a2);
+    DYF_LAUNCHED("phase_segments_dev_kernel");
+    return DYF_OK;
 }
 
 // ---- e^{iH(y, z_k)} for one line along x_lab (the analytic kernel's

@@ -85,6 +85,34 @@ def laue_rocking_scan(xtal, n_s=64, n_seg=8, rng=None):
     return grid, segs, s
 
 
+def ensemble_batch(n=256, rng=None, dx=0.25e-6):
+    """The batched-ensembles configuration: n instances, half diamond
+    (400) at 17 keV (U_lab = I), half (220) at 12 keV with
+    U_lab = lab_frame((0,0,1), (2,2,0)) (g along x), f' = 0.003,
+    f'' = 0.002 for both; t ~ U(50, 300) um, beam ~ U(60, 120) um,
+    1-40 segments of the <110> family, 16-128 angles uniform in
+    [-200, +200] urad.  Returns a list of (xtal, grid, segments, s_dev)."""
+    from .crystal import DIAMOND, CrystalParams, lab_frame
+    from .laue.grid import LaueGrid
+    rng = np.random.default_rng(SEED) if rng is None else rng
+    x400 = CrystalParams.from_reflection(DIAMOND, (4, 0, 0), 17.0,
+                                         f_prime=0.003, f_dprime=0.002)
+    x220 = CrystalParams.from_reflection(
+        DIAMOND, (2, 2, 0), 12.0, U_lab=lab_frame((0, 0, 1), (2, 2, 0)),
+        f_prime=0.003, f_dprime=0.002)
+    out = []
+    for i in range(n):
+        xtal = x400 if i < n // 2 else x220
+        t = rng.uniform(50e-6, 300e-6)
+        beam = rng.uniform(60e-6, 120e-6)
+        grid = LaueGrid(xtal, t, dx, beam)
+        segs = random_segments(rng, xtal, grid, int(rng.integers(1, 41)))
+        n_s = int(rng.integers(16, 129))
+        s = deviation_from_urad(np.linspace(-200, 200, n_s), xtal)
+        out.append((xtal, grid, segs, s))
+    return out
+
+
 def bragg_surface(xtal, case="B", n_planes=32, n_s=41):
     from .bragg.geometry import BraggConfig, bragg_g_vector
     cfg = BraggConfig(xtal, t_crystal=20e-6, dx=0.5e-6, half_x=32e-6)

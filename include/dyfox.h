@@ -142,6 +142,49 @@ int dyf_laue_march(const dyf_laue_args *a, int mode, int prec,
                    const dyf_line_params *line, const double *s_dev_dev,
                    int n_e, void *coef_dev, void *out_dev, void *stream);
 
+/* ---- Batched Laue solves (north-star: persistent launch) ------------- */
+/* One instance: its grid and crystal, segments (host), angles and output.
+ * Batches of these run as one persistent march (no reference
+ * counterpart: the reference launches one grid per solve,
+ * laue/solver.py:62-90).                                                  */
+typedef struct {
+    dyf_laue_args grid;
+    const double *segs;       /* (nseg, 9) host: ra, rb, b per segment   */
+    int32_t nseg, n_e;
+    double g_vec[3];
+    double nu, a_core;
+    const double *s_dev;      /* n_e doubles, device                     */
+    void *out;                /* (n_e, Ny, Nx) complex, device           */
+} dyf_laue_instance;
+
+typedef struct {
+    int32_t waves_per_block;  /* 1, 2 or 4 wavefronts per workgroup      */
+    int32_t blocks_per_cu;    /* persistent workgroups per CU            */
+} dyf_march_tune;
+
+/* Workspace (e^{iH} + coefficient tables) and meta (segments,
+ * descriptors, queue counters) sizes of a batch.                          */
+int dyf_laue_batch_bytes(const dyf_laue_instance *inst, int n, int prec,
+                         size_t *work_bytes, size_t *meta_bytes);
+/* Upload segment parameters and instance descriptors (synchronous).      */
+int dyf_laue_batch_prepare(const dyf_laue_instance *inst, int n, int prec,
+                           void *work_dev, size_t work_bytes, void *meta_dev,
+                           size_t meta_bytes);
+/* Enqueue the batch: stages bit 1 = e^{iH} tables, 2 = coefficients and
+ * out-of-FoV zero rows, 4 = the persistent march.  Kernels only, so the
+ * call can be captured in a graph (dyf_graph_begin/end).                 */
+int dyf_laue_batch_run(const dyf_laue_instance *inst, int n, int prec,
+                       void *work_dev, void *meta_dev,
+                       const dyf_march_tune *tune, int stages, void *stream);
+/* hipGraph capture of the work enqueued on `stream` between begin and
+ * end; launch replays it.                                                 */
+int dyf_graph_begin(void *stream);
+int dyf_graph_end(void *stream, void **graph_exec);
+int dyf_graph_launch(void *graph_exec, void *stream);
+int dyf_graph_destroy(void *graph_exec);
+int dyf_stream_create(void **stream);
+int dyf_stream_destroy(void *stream);
+
 /* ---- Bragg shooting solve (bragg/solver.py:28-60, 135-379) ---------- */
 typedef struct {
     double s0r, s0i, shr, shi, sbr, sbi;  /* sigma_0, sigma_h, sigma_hbar */

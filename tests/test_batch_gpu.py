@@ -1,0 +1,92 @@
+"""Batched uneven ensembles (persistent march, graph replay, autotuner).
+
+Every instance of a batch must come out exactly as its own
+solve_dislocation call (same kernels, same arithmetic), whatever the
+grouping, launch shape or graph replay; a (220) at 12 keV instance with
+g along x is also checked against the oracle.
+"""
+
+import numpy as np
+import pytest
+
+from dyfox_amd.backend import FP64, MIXED, to_numpy
+from dyfox_amd.crystal import DIAMOND, CrystalParams, lab_frame
+from dyfox_amd.laue import LaueGrid, solve_dislocation
+from dyfox_amd.laue.batch import EnsembleSolver
+from dyfox_amd.units import deviation_from_urad
+from dyfox_amd.workloads import random_segments
+from oracle import laue as OL
+from tests.helpers import oracle_exit
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def x220():
+    return CrystalParams.from_reflection(
+        DIAMOND, (2, 2, 0), 12.0, U_lab=lab_frame((0, 0, 1), (2, 2, 0)),
+        f_prime=0.003, f_dprime=0.002)
+
+
+def small_batch(diamond400, x220, n=6, seed=7):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        xtal = diamond400 if i % 2 == 0 else x220
+        t = rng.uniform(15e-6, 60e-6)
+        grid = LaueGrid(xtal, t, 0.25e-6, rng.uniform(3e-6, 6e-6))
+        segs = random_segments(rng, xtal, grid, int(rng.integers(1, 5)))
+        n_s = int(rng.integers(2, 7))
+        out.append((xtal, grid, segs,
+                    deviation_from_urad(np.linspace(-200, 200, n_s), xtal)))
+    return out
+
+
+def test_batch_equals_single_solves(dev, diamond400, x220):
+    inst = small_batch(diamond400, x220)
+    for prec in (FP64, MIXED):
+        solver = EnsembleSolver(inst, precision=prec)
+        outs = [to_numpy(o) for o in solver.run()]
+        for (xtal, grid, segs, s), got in zip(inst, outs):
+            want = to_numpy(solve_dislocation(segs, s, grid, xtal,
+                                              precision=prec))
+            assert got.shape == want.shape and got.dtype == want.dtype
+            scale = np.abs(want).max()
+            assert np.abs(got - want).max() <= 1e-13 * scale, prec
+        # graph replay after other eager work gives the same answer again
+        solve_dislocation(inst[0][2], inst[0][3], inst[0][1], inst[0][0])
+        again = [to_numpy(o) for o in solver.run()]
+        for a, b in zip(outs, again):
+            assert np.array_equal(a, b)
+        solver.close()
+
+
+def test_220_instance_matches_oracle(dev, x220):
+    grid = LaueGrid(x220, 40e-6, 0.25e-6, 4e-6)
+    rng = np.random.default_rng(3)
+    segs = random_segments(rng, x220, grid, 3)
+    s = deviation_from_urad(np.array([-150.0, 0.0, 90.0]), x220)
+    solver = EnsembleSolver([(x220, grid, segs, s)])
+    got = to_numpy(solver.run()[0])
+    want = oracle_exit(segs, s, grid, x220)
+    for r in OL.error_report(got, want):
+        assert r["rel_rms"] < 1e-10 and r["max_abs"] < 1e-10
+    solver.close()
+
+
+def test_groups_and_launch_shapes_change_nothing(dev, diamond400, x220):
+    inst = small_batch(diamond400, x220, n=5, seed=11)
+    ref = EnsembleSolver(inst)
+    want = [to_numpy(o) for o in ref.run()]
+    timings = ref.autotune(repeats=1)
+    assert set(timings) and ref.tune in timings
+    again = [to_numpy(o) for o in ref.run()]
+    for a, b in zip(want, again):
+        assert np.array_equal(a, b)
+    ref.close()
+    small = EnsembleSolver(inst, budget_bytes=1, use_graph=False)
+    assert len(small.groups) == len(inst)
+    got = [to_numpy(o) for o in small.run()]
+    for a, b in zip(want, got):
+        assert np.array_equal(a, b)
+    small.close()
