@@ -61,7 +61,8 @@ class MarchTune(C.Structure):
 class BraggArgs(C.Structure):
     _fields_ = [(n, C.c_double) for n in
                 ("s0r", "s0i", "shr", "shi", "sbr", "sbi", "ds")] + \
-               [(n, C.c_int32) for n in ("Nz", "Nx", "nyc", "reserved")]
+               [(n, C.c_int32) for n in ("Nz", "Nx", "nyc",
+                                         "per_system_phase")]
 
 
 class GmresOpts(C.Structure):
@@ -121,6 +122,33 @@ _SIGNATURES = {
     "dyf_graph_destroy": [vp],
     "dyf_stream_create": [C.POINTER(vp)],
     "dyf_stream_destroy": [vp],
+    "dyf_nn_gemm": [vp, C.c_longlong, C.c_longlong, vp, C.c_longlong,
+                    C.c_longlong, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                    vp, vp, C.c_longlong, C.c_int, vp],
+    "dyf_nn_im2col3": [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                       C.c_int, vp, vp],
+    "dyf_nn_col2im3": [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, C.c_int,
+                       C.c_int, vp],
+    "dyf_nn_slice": [vp, C.c_longlong, C.c_int, C.c_int, C.c_int, vp, vp],
+    "dyf_nn_relu_grad": [vp, vp, C.c_longlong, C.c_int, C.c_int, C.c_int,
+                         C.c_int, vp, vp],
+    "dyf_nn_bias_grad": [vp, C.c_longlong, C.c_int, vp, vp],
+    "dyf_nn_pool": [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                    vp, vp],
+    "dyf_nn_pool_grad": [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, C.c_int,
+                         C.c_int, vp],
+    "dyf_nn_up": [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, C.c_int,
+                  C.c_int, vp],
+    "dyf_nn_up_grad": [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                       C.c_int, vp, vp],
+    "dyf_nn_mse": [vp, vp, C.c_longlong, vp, vp, vp],
+    "dyf_nn_adam": [vp, vp, vp, vp, vp, C.c_longlong, C.c_float, C.c_float,
+                    C.c_float, C.c_float, C.c_int, vp],
+    "dyf_nn_to_bf16": [vp, C.c_longlong, vp, vp],
+    "dyf_nn_fill": [vp, C.c_longlong, C.c_float, vp],
+    "dyf_nn_bragg_features": [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                              C.c_int, C.c_int, vp, vp, vp, C.c_int, C.c_int,
+                              C.c_int, vp],
     "dyf_bragg_phase": [dp, C.c_int, dp, C.c_double, C.c_double,
                         C.POINTER(VolumeDesc), dp, C.c_int, vp, vp],
     "dyf_bragg_march": [C.POINTER(BraggArgs), vp, C.c_int, vp, vp, vp, vp,

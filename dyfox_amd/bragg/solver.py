@@ -48,19 +48,20 @@ def perfect_symbol(e0, eg, ds, Nz, Nx, xtal):
     return T21, T22
 
 
-def bragg_args(cfg, xtal, nyc):
-    return N.BraggArgs(
+def bragg_args(cfg, xtal, nyc, per_system=False):
+    return N.BraggArgs(per_system_phase=int(bool(per_system)),
         s0r=float(np.real(xtal.sig0)), s0i=float(np.imag(xtal.sig0)),
         shr=float(np.real(xtal.sig_h)), shi=float(np.imag(xtal.sig_h)),
         sbr=float(np.real(xtal.sig_hbar)), sbi=float(np.imag(xtal.sig_hbar)),
-        ds=float(cfg.ds), Nz=int(cfg.Nz), Nx=int(cfg.Nx), nyc=int(nyc),
-        reserved=0)
+        ds=float(cfg.ds), Nz=int(cfg.Nz), Nx=int(cfg.Nx), nyc=int(nyc))
 
 
-def chunk_phase(seg_list, cfg, xtal, g_vec, y_values):
-    """e^{iH} on the planes y_values: device (Nz, ny, Nx) complex128."""
+def chunk_phase(seg_list, cfg, xtal, g_vec, y_values, out=None):
+    """e^{iH} on the planes y_values: device (Nz, ny, Nx) complex128
+    (written into `out` when given, e.g. one system's slot of a stack)."""
     ys = np.ascontiguousarray(y_values, dtype=np.float64)
-    E = N.DeviceArray.empty((cfg.Nz, len(ys), cfg.Nx), np.complex128)
+    E = out if out is not None else N.DeviceArray.empty(
+        (cfg.Nz, len(ys), cfg.Nx), np.complex128)
     segs = pack_segments(seg_list)
     g = np.ascontiguousarray(g_vec, dtype=np.float64)
     vol = N.VolumeDesc(int(cfg.Nx), int(cfg.Nz), float(cfg.dx), float(cfg.dz))
@@ -120,8 +121,9 @@ def gmres_batch(cfg, xtal, s_dev, E, incident, *, tol, restart, maxiter,
     (residual, iterations, code)."""
     s = np.atleast_1d(np.asarray(s_dev, np.float64))
     n = len(s)
-    nyc = E.shape[1]
-    a = bragg_args(cfg, xtal, nyc)
+    per = E.ndim == 4               # (n, Nz, nyc, Nx): one table per system
+    nyc = E.shape[-2]
+    a = bragg_args(cfg, xtal, nyc, per)
     nb = C.c_size_t()
     N.call("dyf_bragg_workspace_bytes", C.byref(a), n, int(restart),
            C.byref(nb))
@@ -143,7 +145,7 @@ def gmres_batch(cfg, xtal, s_dev, E, incident, *, tol, restart, maxiter,
 
 def exit_wave(cfg, xtal, E, v):
     """v e^{-iH(z=0)} on the host, (n_s, nyc, Nx)."""
-    a = bragg_args(cfg, xtal, E.shape[1])
+    a = bragg_args(cfg, xtal, E.shape[-2], E.ndim == 4)
     out = N.DeviceArray.empty(v.shape, np.complex128)
     N.call("dyf_bragg_exit_wave", C.byref(a), v.shape[0], C.c_void_p(E.ptr),
            C.c_void_p(v.ptr), C.c_void_p(out.ptr), None)

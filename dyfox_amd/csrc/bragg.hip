@@ -49,6 +49,7 @@ __device__ __forceinline__ cd cdiv(cd a, cd b) {
 struct BraggK {
     double s0r, s0i, shr, shi, sbr, sbi, ds;
     int Nz, Nx, nyc, nsys;
+    long long estride;   // elements between systems' e^{iH} tables (0: shared)
 };
 
 static BraggK make_k(const dyf_bragg_args *a, int nsys) {
@@ -56,6 +57,7 @@ static BraggK make_k(const dyf_bragg_args *a, int nsys) {
     k.s0r = a->s0r, k.s0i = a->s0i, k.shr = a->shr, k.shi = a->shi;
     k.sbr = a->sbr, k.sbi = a->sbi, k.ds = a->ds;
     k.Nz = a->Nz, k.Nx = a->Nx, k.nyc = a->nyc, k.nsys = nsys;
+    k.estride = a->per_system_phase ? (long long)a->Nz * a->nyc * a->Nx : 0;
     return k;
 }
 
@@ -75,6 +77,7 @@ __global__ __launch_bounds__(kMarchThreads) void bragg_march_kernel(
     extern __shared__ cd lds_b[];
     const int s = blockIdx.x / k.nyc, j = blockIdx.x % k.nyc;
     if (active && !active[s]) return;
+    E += (size_t)s * k.estride;
     const int Nx = k.Nx, tid = threadIdx.x;
     cd *D0[2] = {lds_b, lds_b + 2 * Nx};
     cd *Dg[2] = {lds_b + Nx, lds_b + 3 * Nx};
@@ -476,11 +479,13 @@ __global__ void lsq_kernel(int nsys, int restart, const cd *H, cd *S, cd *Y,
 
 __global__ void exit_wave_kernel(const cd *__restrict__ v,
                                  const cd *__restrict__ E, int nyc, int Nx,
-                                 size_t n, cd *__restrict__ out) {
+                                 long long estride, size_t n,
+                                 cd *__restrict__ out) {
+    const size_t plane = (size_t)nyc * Nx;
     for (size_t t = (size_t)blockIdx.x * 256 + threadIdx.x; t < n;
          t += (size_t)gridDim.x * 256) {
-        const size_t pix = t % ((size_t)nyc * Nx);
-        out[t] = cconjmul(E[pix], v[t]);
+        const size_t s = t / plane, pix = t % plane;
+        out[t] = cconjmul(E[s * (size_t)estride + pix], v[t]);
     }
 }
 
@@ -604,8 +609,10 @@ extern "C" int dyf_bragg_exit_wave(const dyf_bragg_args *a, int nsys,
     if (rc) return rc;
     const size_t n = (size_t)nsys * a->nyc * a->Nx;
     const int blocks = (int)std::min<size_t>((n + 255) / 256, 8192);
+    const long long estride =
+        a->per_system_phase ? (long long)a->Nz * a->nyc * a->Nx : 0;
     exit_wave_kernel<<<blocks, 256, 0, (hipStream_t)stream>>>(
-        (const cd *)v, (const cd *)E, a->nyc, a->Nx, n, (cd *)out);
+        (const cd *)v, (const cd *)E, a->nyc, a->Nx, estride, n, (cd *)out);
     DYF_LAUNCHED("exit_wave_kernel");
     return DYF_OK;
 }

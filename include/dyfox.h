@@ -189,7 +189,11 @@ int dyf_stream_destroy(void *stream);
 typedef struct {
     double s0r, s0i, shr, shi, sbr, sbi;  /* sigma_0, sigma_h, sigma_hbar */
     double ds;                            /* path step dz / sin(theta_B)  */
-    int32_t Nz, Nx, nyc, reserved;        /* depth planes, columns, y-planes */
+    int32_t Nz, Nx, nyc;                  /* depth planes, columns, y-planes */
+    int32_t per_system_phase;  /* 0: one E (Nz, nyc, Nx) shared by all
+                                  systems; 1: E is (nsys, Nz, nyc, Nx), one
+                                  table per system (different dislocation
+                                  fields on the same grid)              */
 } dyf_bragg_args;
 
 typedef struct {
@@ -238,6 +242,52 @@ int dyf_bragg_solve(const dyf_bragg_args *a, const double *s_dev_dev,
 /* Exit wave v e^{-iH(z=0)} (bragg/solver.py:365).                        */
 int dyf_bragg_exit_wave(const dyf_bragg_args *a, int nsys, const void *E_dev,
                         const void *v_dev, void *out_dev, void *stream);
+
+/* ---- Learned warm start: BF16-on-MFMA network kernels --------------- */
+/* No counterpart in the reference (north-star addition).  Tensors are
+ * NHWC; bf16 is stored as uint16 (bfloat16 bits).  A layer reads a channel
+ * slice [off, off+C) of a buffer with Ctot channels.                     */
+/* C(i,j) = sum_k A(i,k) B(j,k), A(i,k) = A[i a_i + k a_k] (bf16),
+ * B(j,k) = B[j b_j + k b_k] (bf16), FP32 accumulation on MFMA.  mode 0:
+ * bf16 store of relu?(acc + bias) at C[i c_i + j]; 1: fp32 store
+ * (+ bias); 2: fp32 atomic add (k_splits > 1 allowed).                   */
+int dyf_nn_gemm(const void *A, long long a_i, long long a_k, const void *B,
+                long long b_j, long long b_k, int M, int N, int K, int mode,
+                int relu, const float *bias, void *C, long long c_i,
+                int k_splits, void *stream);
+int dyf_nn_im2col3(const void *X, int B, int H, int W, int Ctot, int off,
+                   int C, void *col, void *stream);
+int dyf_nn_col2im3(const float *dcol, int B, int H, int W, int C, float *dX,
+                   int Ctot, int off, void *stream);
+int dyf_nn_slice(const void *X, long long P, int Ctot, int off, int C,
+                 void *out, void *stream);
+int dyf_nn_relu_grad(const float *dY, const void *Y, long long P, int Ctot,
+                     int off, int C, int relu, void *dZ, void *stream);
+int dyf_nn_bias_grad(const void *dZ, long long P, int C, float *db,
+                     void *stream);
+int dyf_nn_pool(const void *X, int B, int H, int W, int Ctot, int off, int C,
+                void *Y, void *stream);
+int dyf_nn_pool_grad(const float *dY, int B, int H, int W, int C, float *dX,
+                     int Ctot, int off, void *stream);
+int dyf_nn_up(const void *X, int B, int H, int W, int C, void *Y, int Ctot,
+              int off, void *stream);
+int dyf_nn_up_grad(const float *dY, int B, int H, int W, int C, int Ctot,
+                   int off, float *dX, void *stream);
+int dyf_nn_mse(const float *out, const float *target, long long n,
+               float *grad, float *loss, void *stream);
+int dyf_nn_adam(float *w, const float *g, float *m, float *v, void *w_bf16,
+                long long n, float lr, float b1, float b2, float eps, int t,
+                void *stream);
+int dyf_nn_to_bf16(const float *x, long long n, void *y, void *stream);
+int dyf_nn_fill(float *x, long long n, float v, void *stream);
+/* Network input from Bragg e^{iH} stacks (nsys, Nz, nyc, Nx) complex128:
+ * Re/Im of the top nk depth slices on columns [x0, x0+Wn), plus the
+ * normalised s_dev and thickness planes, written as bf16 NHWC rows
+ * y_off.. of samples first..first+nsys of a (*, Hn, Wn, 2nk+2) tensor.  */
+int dyf_nn_bragg_features(const void *E, int nsys, int Nz, int nyc, int Nx,
+                          int x0, int Wn, int nk, const float *s_norm,
+                          const float *t_norm, void *out, int Hn, int y_off,
+                          int first, void *stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,114 @@
+"""The warm-start network's device kernels against the NumPy oracle, and
+the warm start against a cold solve.
+
+Bars: the MFMA GEMM equals a float64 product of the same bf16 operands to
+1e-5 relative (fp32 accumulation); the network's output and gradients
+agree with the FP64 oracle evaluated on the same bf16-rounded weights and
+inputs to the level bf16 activations allow (relative 3e-2, gradient
+cosine similarity > 0.995); a warm-started solve reaches the same answer
+at the same residual.
+"""
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from dyfox_amd import _native as N
+from dyfox_amd.warmstart.net import (BUFFERS, SPEC, WarmStartNet,
+                                     from_bf16_bits, init_params,
+                                     to_bf16_bits)
+from oracle import network as ON
+
+pytestmark = pytest.mark.gpu
+
+
+def bf(a):
+    return from_bf16_bits(to_bf16_bits(a)).astype(np.float64)
+
+
+@pytest.mark.parametrize("M,N_,K,trans", [(70, 45, 300, False),
+                                          (128, 64, 32, False),
+                                          (33, 200, 517, True)])
+def test_gemm_matches_a_float64_product(dev, rng, M, N_, K, trans):
+    A = rng.normal(size=(M, K)).astype(np.float32)
+    B = rng.normal(size=(N_, K)).astype(np.float32)
+    want = bf(A) @ bf(B).T
+    if trans:          # operands stored k-major: A(i,k) at k*M + i
+        Ad = N.DeviceArray.from_host(to_bf16_bits(A.T.copy()))
+        Bd = N.DeviceArray.from_host(to_bf16_bits(B.T.copy()))
+        strides = (1, M, 1, N_)
+    else:
+        Ad = N.DeviceArray.from_host(to_bf16_bits(A))
+        Bd = N.DeviceArray.from_host(to_bf16_bits(B))
+        strides = (K, 1, K, 1)
+    Cd = N.DeviceArray.zeros((M * N_,), np.float32)
+    for splits, mode in ((1, 1), (3, 2)):
+        if mode == 2:
+            Cd = N.DeviceArray.zeros((M * N_,), np.float32)
+        N.call("dyf_nn_gemm", C.c_void_p(Ad.ptr), strides[0], strides[1],
+               C.c_void_p(Bd.ptr), strides[2], strides[3], M, N_, K, mode, 0,
+               None, C.c_void_p(Cd.ptr), N_, splits, None)
+        got = Cd.get().reshape(M, N_)
+        assert np.abs(got - want).max() <= 1e-5 * np.abs(want).max()
+
+
+def test_forward_and_gradients_match_the_oracle(dev, rng):
+    h0, w0, B = 16, 32, 2
+    p = init_params(rng)
+    layout, _ = ON.param_layout(SPEC)
+    i, wo, (co, k), bo, _ = layout[-1]
+    p[wo:wo + co * k] *= 10
+    p[[l[3] for l in layout]] = rng.normal(scale=0.05, size=len(layout))
+    pb = bf(p)
+    net = WarmStartNet(B, params=pb.astype(np.float32), h0=h0, w0=w0)
+    x = bf(rng.normal(size=(B, h0, w0, 34)))
+    t = rng.normal(size=(B, h0, w0, 2))
+    xd = N.DeviceArray.from_host(to_bf16_bits(x))
+    td = N.DeviceArray.from_host(t.astype(np.float32))
+    out = net.forward(xd).get().reshape(B, h0, w0, 2)
+    loss = float(net.backward(td).get()[0])
+    g = net.g.get().astype(np.float64)
+    wl, wg, wout = ON.loss_and_grad(SPEC, BUFFERS, pb, x, t, h0, w0)
+    rel_out = np.abs(out - wout).max() / np.abs(wout).max()
+    cos = g @ wg / (np.linalg.norm(g) * np.linalg.norm(wg))
+    rel_g = np.linalg.norm(g - wg) / np.linalg.norm(wg)
+    print(f"out rel {rel_out:.2e} loss {loss:.5f}/{wl:.5f} grad cos "
+          f"{cos:.6f} rel {rel_g:.2e}")
+    assert rel_out < 3e-2
+    assert abs(loss - wl) < 2e-2 * wl
+    assert cos > 0.995 and rel_g < 0.1
+    for li, (_, wo2, (c2, k2), bo2, _) in enumerate(layout):
+        gw, ww = g[wo2:wo2 + c2 * k2], wg[wo2:wo2 + c2 * k2]
+        assert gw @ ww / (np.linalg.norm(gw) * np.linalg.norm(ww)) > 0.98, li
+
+
+def test_training_reduces_the_loss(dev, rng):
+    h0, w0, B = 16, 32, 4
+    net = WarmStartNet(B, h0=h0, w0=w0, seed=3)
+    x = rng.normal(size=(B, h0, w0, 34))
+    t = np.stack([x[..., 0] * 0.5 - x[..., 3], x[..., 1] * x[..., 2]], -1)
+    xd = N.DeviceArray.from_host(to_bf16_bits(x))
+    td = N.DeviceArray.from_host(t.astype(np.float32))
+    losses = []
+    for _ in range(60):
+        net.forward(xd)
+        losses.append(float(net.backward(td).get()[0]))
+        net.step(2e-3)
+    assert losses[-1] < 0.5 * losses[0]
+
+
+def test_warm_start_converges_to_the_cold_answer(dev, diamond400):
+    from dyfox_amd.warmstart import data
+    rng = np.random.default_rng(5)
+    inst = data.draw_instances(40, rng, diamond400, t_range=(20e-6, 22.5e-6),
+                               max_seg=4)
+    ds = data.build_dataset(diamond400, inst, max_sys=40)
+    assert (ds.resid <= 1e-10).all()
+    model, losses = data.train(ds, 32, batch=8, epochs=3)
+    assert np.isfinite(losses).all()
+    rep = data.evaluate(diamond400, inst, ds, model, list(range(32, 40)))
+    print(rep)
+    assert rep["warm_worst_residual"] <= 1e-10
+    assert rep["cold_worst_residual"] <= 1e-10
+    assert rep["max_abs_diff_warm_vs_cold_v"] < 1e-8
