@@ -62,7 +62,8 @@ class BraggArgs(C.Structure):
     _fields_ = [(n, C.c_double) for n in
                 ("s0r", "s0i", "shr", "shi", "sbr", "sbi", "ds")] + \
                [(n, C.c_int32) for n in ("Nz", "Nx", "nyc",
-                                         "per_system_phase")]
+                                         "per_system_phase", "plane_systems",
+                                         "reserved")]
 
 
 class GmresOpts(C.Structure):

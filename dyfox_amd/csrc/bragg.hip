@@ -49,15 +49,27 @@ __device__ __forceinline__ cd cdiv(cd a, cd b) {
 struct BraggK {
     double s0r, s0i, shr, shi, sbr, sbi, ds;
     int Nz, Nx, nyc, nsys;
-    long long estride;   // elements between systems' e^{iH} tables (0: shared)
+    int tpl;             // y-planes per depth slice of the e^{iH} table
+    int sper;            // systems per table (planes of it when each
+                         // system is one plane, else 1)
+    long long tstride;   // elements between tables (0: one shared table)
 };
+
+// First element of system s's e^{iH} table at depth 0, plane 0; depth
+// slices are tpl * Nx apart.
+__host__ __device__ inline long long table_offset(const BraggK &k, int s) {
+    return (long long)(s / k.sper) * k.tstride +
+           (long long)(s % k.sper) * k.nyc * k.Nx;
+}
 
 static BraggK make_k(const dyf_bragg_args *a, int nsys) {
     BraggK k;
     k.s0r = a->s0r, k.s0i = a->s0i, k.shr = a->shr, k.shi = a->shi;
     k.sbr = a->sbr, k.sbi = a->sbi, k.ds = a->ds;
     k.Nz = a->Nz, k.Nx = a->Nx, k.nyc = a->nyc, k.nsys = nsys;
-    k.estride = a->per_system_phase ? (long long)a->Nz * a->nyc * a->Nx : 0;
+    k.sper = a->plane_systems > 0 ? a->plane_systems : 1;
+    k.tpl = a->plane_systems > 0 ? a->plane_systems : a->nyc;
+    k.tstride = a->per_system_phase ? (long long)a->Nz * k.tpl * a->Nx : 0;
     return k;
 }
 
@@ -77,7 +89,7 @@ __global__ __launch_bounds__(kMarchThreads) void bragg_march_kernel(
     extern __shared__ cd lds_b[];
     const int s = blockIdx.x / k.nyc, j = blockIdx.x % k.nyc;
     if (active && !active[s]) return;
-    E += (size_t)s * k.estride;
+    E += table_offset(k, s);
     const int Nx = k.Nx, tid = threadIdx.x;
     cd *D0[2] = {lds_b, lds_b + 2 * Nx};
     cd *Dg[2] = {lds_b + Nx, lds_b + 3 * Nx};
@@ -94,7 +106,7 @@ __global__ __launch_bounds__(kMarchThreads) void bragg_march_kernel(
     const cd B = {k.ds * k.shr, k.ds * k.shi};   // ds sigma_h
     __syncthreads();
     int cur = 0;
-    const size_t plane = (size_t)k.nyc * Nx;
+    const size_t plane = (size_t)k.tpl * Nx;
     cd en[kMaxPerThread];
 #pragma unroll
     for (int m = 0; m < kMaxPerThread; ++m) {
@@ -478,14 +490,13 @@ __global__ void lsq_kernel(int nsys, int restart, const cd *H, cd *S, cd *Y,
 }
 
 __global__ void exit_wave_kernel(const cd *__restrict__ v,
-                                 const cd *__restrict__ E, int nyc, int Nx,
-                                 long long estride, size_t n,
+                                 const cd *__restrict__ E, BraggK k, size_t n,
                                  cd *__restrict__ out) {
-    const size_t plane = (size_t)nyc * Nx;
+    const size_t plane = (size_t)k.nyc * k.Nx;
     for (size_t t = (size_t)blockIdx.x * 256 + threadIdx.x; t < n;
          t += (size_t)gridDim.x * 256) {
         const size_t s = t / plane, pix = t % plane;
-        out[t] = cconjmul(E[s * (size_t)estride + pix], v[t]);
+        out[t] = cconjmul(E[table_offset(k, (int)s) + pix], v[t]);
     }
 }
 
@@ -563,6 +574,12 @@ static int check_bragg(const dyf_bragg_args *a, int nsys) {
         return fail(DYF_EINVAL, "Nx=%d exceeds %d", a->Nx,
                     kMarchThreads * kMaxPerThread);
     if (!(a->ds > 0.0)) return fail(DYF_EINVAL, "ds must be positive");
+    if (a->plane_systems < 0 ||
+        (a->plane_systems > 0 &&
+         (a->nyc != 1 || nsys % a->plane_systems != 0)))
+        return fail(DYF_EINVAL,
+                    "plane_systems=%d needs nyc=1 and nsys (%d) a multiple",
+                    a->plane_systems, nsys);
     return DYF_OK;
 }
 
@@ -609,10 +626,8 @@ extern "C" int dyf_bragg_exit_wave(const dyf_bragg_args *a, int nsys,
     if (rc) return rc;
     const size_t n = (size_t)nsys * a->nyc * a->Nx;
     const int blocks = (int)std::min<size_t>((n + 255) / 256, 8192);
-    const long long estride =
-        a->per_system_phase ? (long long)a->Nz * a->nyc * a->Nx : 0;
     exit_wave_kernel<<<blocks, 256, 0, (hipStream_t)stream>>>(
-        (const cd *)v, (const cd *)E, a->nyc, a->Nx, estride, n, (cd *)out);
+        (const cd *)v, (const cd *)E, make_k(a, nsys), n, (cd *)out);
     DYF_LAUNCHED("exit_wave_kernel");
     return DYF_OK;
 }

@@ -121,11 +121,15 @@ def _norms(xtal, inst):
 
 
 def solve_group(xtal, idx, instances, *, tol=1e-10, warm=None, ds_out=None,
-                restart=None, maxiter=25):
+                restart=None, maxiter=25, planes=True):
     """Solve the instances `idx` (same grid) chunk by chunk in batched
     GMRES; optionally write network features into `ds_out`.  `warm`:
-    (len(idx), 64, Nx) initial guesses or None.  Returns v (len, 64, Nx),
-    per-instance total iterations, worst residual per instance."""
+    (len(idx), 64, Nx) initial guesses or None.  planes=True gives every
+    y-plane its own Krylov space (restart 40), planes=False solves each
+    32-plane chunk in one (restart 192, the reference's chunk sizing).
+    Returns v (len, 64, Nx), per-instance plane-iterations (matvecs
+    counted per plane: iterations x planes in the system, summed), the
+    worst residual per instance and the grid."""
     cfg = BraggConfig(xtal, t_crystal=instances[idx[0]]["t"], dx=DX,
                       half_x=HALF_X)
     g = bragg_g_vector(xtal)
@@ -137,7 +141,7 @@ def solve_group(xtal, idx, instances, *, tol=1e-10, warm=None, ds_out=None,
     res = np.zeros(n)
     inc = cfg.incident(n=1)[0]
     x0c = (cfg.Nx - W0) // 2
-    r = restart or max(40, min(256, 6 * CHUNK))
+    r = restart or (40 if planes else max(40, min(256, 6 * CHUNK)))
     for c0 in range(0, NPLANES, CHUNK):
         E = N.DeviceArray.empty((n, cfg.Nz, CHUNK, cfg.Nx), np.complex128)
         for j, i in enumerate(idx):
@@ -159,11 +163,13 @@ def solve_group(xtal, idx, instances, *, tol=1e-10, warm=None, ds_out=None,
             x0 = N.DeviceArray.from_host(
                 np.ascontiguousarray(warm[:, c0:c0 + CHUNK]))
         v, info = gmres_batch(cfg, xtal, s, E, inc, tol=tol, restart=r,
-                              maxiter=maxiter, x0=x0)
+                              maxiter=maxiter, x0=x0, planes=planes)
         v_all[:, c0:c0 + CHUNK] = v.get()
+        per = CHUNK if planes else 1
         for j in range(n):
-            its[j] += info[j][1]
-            res[j] = max(res[j], info[j][0])
+            for k in info[j * per:(j + 1) * per]:
+                its[j] += k[1] * (1 if planes else CHUNK)
+                res[j] = max(res[j], k[0])
         del E
     return v_all, its, res, cfg
 

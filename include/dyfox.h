@@ -194,6 +194,13 @@ typedef struct {
                                   systems; 1: E is (nsys, Nz, nyc, Nx), one
                                   table per system (different dislocation
                                   fields on the same grid)              */
+    int32_t plane_systems;     /* 0: each system is nyc planes of its table.
+                                  P > 0 (needs nyc = 1): each system is ONE
+                                  plane of a P-plane table, system s = plane
+                                  s % P of table s / P (or of the shared
+                                  table), so every plane gets its own Krylov
+                                  space in dyf_bragg_solve               */
+    int32_t reserved;          /* 0 */
 } dyf_bragg_args;
 
 typedef struct {

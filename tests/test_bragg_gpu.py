@@ -10,6 +10,7 @@ closed form; batching and chunking leave the answer unchanged.
 import numpy as np
 import pytest
 
+from dyfox_amd import _native as N
 from dyfox_amd.bragg import (BraggConfig, bragg_g_vector, solve_bragg_stable,
                              solve_surface_batch, threading_segment)
 from dyfox_amd.bragg import solver as BS
@@ -80,6 +81,54 @@ def test_surface_batch_matches_oracle(dev, diamond400, g):
         assert np.abs(got[j] - one).max() < 1e-7 * np.abs(one).max()
 
 
+def test_joint_chunks_follow_scipy_and_agree_with_planes(dev, diamond400,
+                                                         g):
+    """planes="joint" is the reference's chunk (SciPy's iteration count);
+    "independent" reaches the same surface with one Krylov space per
+    plane."""
+    cfg = BraggConfig(diamond400, t_crystal=6e-6, dx=0.2e-6, half_x=2e-6)
+    sd = float(deviation_from_urad(-6.5, diamond400))
+    seg = threading_segment("Edge", +1, diamond400.b_mag, float(g[2]),
+                            y0=-0.3e-6)
+    ys = (np.arange(6) - 3) * cfg.dx
+    joint, ij = solve_surface_batch(seg, sd, cfg, ys, diamond400, g,
+                                    planes="joint")
+    want, winfo = OB.solve_surface_batch(seg, sd, cfg, ys, diamond400, g)
+    assert ij["iterations"] == winfo["iterations"]
+    assert _rel(joint, want) < 1e-8
+    ind, ii = solve_surface_batch(seg, sd, cfg, ys, diamond400, g)
+    assert ii["converged"] and ii["residual"] <= 1e-10
+    assert ii["iterations"] <= ij["iterations"]
+    assert _rel(ind, want) < 1e-8
+
+
+def test_plane_systems_equal_single_plane_solves(dev, diamond400, g):
+    """One system per (table, plane), read in place from per-instance
+    tables, equals solving each plane alone on its own table."""
+    cfg = BraggConfig(diamond400, t_crystal=5e-6, dx=0.2e-6, half_x=2e-6)
+    ys = (np.arange(3) - 1) * cfg.dx
+    segs = [threading_segment("Edge", +1, diamond400.b_mag, float(g[2]),
+                              y0=-0.2e-6),
+            threading_segment("Screw", -1, diamond400.b_mag, float(g[2]),
+                              x0=0.5e-6)]
+    s = deviation_from_urad(np.array([-6.5, 9.0]), diamond400)
+    inc = cfg.incident(n=1)[0]
+    E = N.DeviceArray.empty((2, cfg.Nz, len(ys), cfg.Nx), np.complex128)
+    for j in range(2):
+        BS.chunk_phase(segs[j], cfg, diamond400, g, ys, out=E[j])
+    v, info = BS.gmres_batch(cfg, diamond400, s, E, inc, tol=1e-11,
+                             restart=40, maxiter=20, planes=True)
+    assert len(info) == 2 * len(ys)
+    vh = v.get()
+    for j in range(2):
+        for p, y in enumerate(ys):
+            Ep = BS.chunk_phase(segs[j], cfg, diamond400, g, [y])
+            one, oi = BS.gmres_batch(cfg, diamond400, [s[j]], Ep, inc,
+                                     tol=1e-11, restart=40, maxiter=20)
+            assert info[j * len(ys) + p][1] == oi[0][1]
+            assert _rel(vh[j, p], one.get()[0, 0]) < 1e-12
+
+
 def test_dense_brute_force_on_a_tiny_grid(dev, diamond400, g):
     tiny = BraggConfig(diamond400, 3e-6, 0.2e-6, 1e-6)
     seg = threading_segment("Edge", +1, diamond400.b_mag, float(g[2]))
@@ -138,7 +187,8 @@ def test_retry_and_split_policy_is_followed(dev, diamond400, g):
     ys = (np.arange(4) - 2) * cfg.dx
     out, info = solve_surface_batch(seg, sd, cfg, ys, diamond400, g,
                                     accept=1e-30, restart=4, restart_cap=16,
-                                    per_plane=1, max_depth=1, tries=3)
+                                    per_plane=1, max_depth=1, tries=3,
+                                    planes="joint")
     assert info["retries"] > 0 and info["splits"] == 1
     assert info["unconverged_chunks"] == 2 and info["converged"] is False
     want, winfo = OB.solve_surface_batch(seg, sd, cfg, ys, diamond400, g,

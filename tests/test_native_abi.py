@@ -41,3 +41,4 @@ def test_structs_match_the_header_layout():
     assert ctypes.sizeof(_native.LaueArgs) == 12 * 8 + 4 * 4
     assert ctypes.sizeof(_native.LineParams) == 12 * 8
     assert ctypes.sizeof(_native.VolumeDesc) == 24
+    assert ctypes.sizeof(_native.BraggArgs) == 7 * 8 + 6 * 4
