@@ -82,17 +82,20 @@ def time_laue(N, segs, s, grid, xtal, prec, steps, warmup):
     return wall, phase_ms, march_ms, out
 
 
-def time_bragg(xtal, case, repeats=3):
+def time_bragg(xtal, case, repeats=3, planes="independent"):
     """cfg1: 41 s_dev over one 32-plane chunk, solved as one batched
-    GMRES per repeat (cold start); returns solves/s and per-solve info."""
+    GMRES per repeat (cold start); returns solves/s and per-solve info.
+    planes="joint" is the reference's chunk (one Krylov space for the 32
+    planes of an angle), "independent" one Krylov space per plane."""
     from dyfox_amd.bragg import solve_surface_batch
     from dyfox_amd.workloads import bragg_surface
     cfg, segs, ys, s, g = bragg_surface(xtal, case)
-    solve_surface_batch(segs, s, cfg, ys, xtal, g)          # plans, warm-up
+    solve_surface_batch(segs, s, cfg, ys, xtal, g, planes=planes)  # warm-up
     times = []
     for _ in range(repeats):
         t0 = time.perf_counter()
-        _, infos = solve_surface_batch(segs, s, cfg, ys, xtal, g)
+        _, infos = solve_surface_batch(segs, s, cfg, ys, xtal, g,
+                                       planes=planes)
         times.append(time.perf_counter() - t0)
     dt = float(np.median(times))
     its = [i["iterations"] for i in infos]
@@ -124,6 +127,29 @@ def cpu_baseline(segs, s, grid, xtal, rows=48, n_s=8):
             "sample": f"{rows} marched rows x {n_s} s_dev of the same "
                       f"workload ({work:.3e} pixel-steps, Volterra H of "
                       f"those rows included), {dt:.1f} s, 1 process"}
+
+
+def warm_start_report():
+    """Cold vs learned-warm-start Bragg solves/s from the committed run of
+    `python -m dyfox_amd.warmstart.experiment --train 20000 --test 2000`
+    (about six minutes of GPU time, so it is not repeated here)."""
+    path = os.path.join(ROOT, "profiles", "bragg_warmstart_cfg5.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        r = json.load(f)
+    v = r["variants"]
+    return {"source": "profiles/bragg_warmstart_cfg5.json (committed run)",
+            "n_test": r["n"], "n_train": r["n_train"],
+            "cold_solves_per_s": v["cold"]["solves_per_s"],
+            "warm_solves_per_s": v["network"]["solves_per_s"],
+            "warm_over_cold": v["network"]["speedup_vs_cold"],
+            "perfect_crystal_guess_over_cold":
+                v["perfect"]["speedup_vs_cold"],
+            "exact_1e-3_guess_over_cold":
+                v["exact_eps0.001"]["speedup_vs_cold"],
+            "iterations_per_plane": {k: v[k]["iterations_per_plane_mean"]
+                                     for k in v}}
 
 
 def traffic_from_profile():
@@ -208,7 +234,9 @@ def main():
           flush=True)
 
     bragg = {c: time_bragg(xtal, c) for c in ("A", "B")}
-    print(f"[bench] bragg cfg1 B {bragg['B']['solves_per_s']:.1f} solves/s",
+    bragg["B_joint_chunk"] = time_bragg(xtal, "B", planes="joint")
+    print(f"[bench] bragg cfg1 B {bragg['B']['solves_per_s']:.1f} solves/s "
+          f"(joint chunk {bragg['B_joint_chunk']['solves_per_s']:.1f})",
           file=sys.stderr, flush=True)
 
     achieved = FLOPS_PER_PIXEL_STEP * work / (m64 * 1e-3) / 1e12
@@ -245,6 +273,7 @@ def main():
                                 "phase_ms": p2, "march_ms": m2,
                                 "pixel_steps": work2},
         "bragg_cfg1_cold": bragg,
+        "bragg_warm_start_cfg5": warm_start_report(),
         "roofline": roof,
     }
     if not args.no_cpu:
