@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true",
                     help="skip the CPU oracle baseline")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the batched-ensembles leg")
     ap.add_argument("--probe", action="store_true",
                     help="for counter runs: one phase build and one march "
                          "per precision, then a 1 GiB copy kernel (the "
@@ -104,6 +106,42 @@ def time_bragg(xtal, case, repeats=3, planes="independent"):
             "worst_residual": max(i["residual"] for i in infos),
             "all_converged": all(i["converged"] for i in infos),
             "grid": f"Nz={cfg.Nz} Nx={cfg.Nx} planes={len(ys)}"}
+
+
+def time_ensembles():
+    """cfg4: 256 uneven Laue instances (two reflections, t 50-300 um,
+    1-40 segments, 16-128 angles) through the persistent batched path:
+    autotuned launch shape, graph captured on the first run, the second
+    (replayed) run timed, FP64 then mixed."""
+    from dyfox_amd.backend import FP64, MIXED
+    from dyfox_amd.laue.batch import EnsembleSolver
+    from dyfox_amd.workloads import ensemble_batch
+    inst = ensemble_batch()
+    out = {}
+    for prec in (FP64, MIXED):
+        t0 = time.perf_counter()
+        sv = EnsembleSolver(inst, precision=prec, keep_outputs=False)
+        setup = time.perf_counter() - t0
+        tim = sv.autotune(repeats=1)
+        sv.run()
+        t0 = time.perf_counter()
+        sv.run()
+        dt = time.perf_counter() - t0
+        ps = sv.pixel_steps
+        out[prec.name] = {"value": ps / dt, "seconds": dt,
+                          "pixel_steps": ps, "groups": len(sv.groups),
+                          "workspace_gb": sv.work.nbytes / 1e9,
+                          "tuned_waves_per_block_blocks_per_cu":
+                              list(sv.tune),
+                          "autotune_ms": {f"{a}x{b}": round(v, 1)
+                                          for (a, b), v in tim.items()},
+                          "setup_s": setup}
+        sv.close()
+        del sv
+        print(f"[bench] cfg4 {prec.name} {ps / dt:.3e} pixel-steps/s "
+              f"({dt:.2f} s)", file=sys.stderr, flush=True)
+    out["n_instances"] = len(inst)
+    return out
 
 
 def cpu_baseline(segs, s, grid, xtal, rows=48, n_s=8):
@@ -273,6 +311,7 @@ def main():
                                 "phase_ms": p2, "march_ms": m2,
                                 "pixel_steps": work2},
         "bragg_cfg1_cold": bragg,
+        "ensemble_cfg4": None if args.no_ensemble else time_ensembles(),
         "bragg_warm_start_cfg5": warm_start_report(),
         "roofline": roof,
     }

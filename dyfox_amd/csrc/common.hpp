@@ -55,7 +55,8 @@ struct SegParams {
     double k_dd;            // -2 pi g.e2 c1 / (4(1-nu))       * (d1^2-d2^2)/r2c
     double k_log;           // -2 pi g.e2 c1 (1-2nu)/(4(1-nu)) * ln r2c
     double edge;            // 1 when b_perp > 1e-20
-    double pad_;
+    double wind;            // g.b rounded, when it is an integer in
+                            // [-8, 8] (within 1e-9); NaN otherwise
 };
 
 // Reduces raw (ra, rb, b) segments to SegParams; segments shorter than
@@ -88,6 +89,55 @@ __device__ __forceinline__ double volterra_h(const SegParams *__restrict__ s,
     return H;
 }
 
+// e^{iH} at (x, y, z) as P e^{iR}: P = prod (w/|w|)^(g.b) over the
+// segments whose g.b is an integer (w = d1 + i d2, so (w/|w|)^n =
+// e^{i n atan2(d2, d1)} without the atan2), R = the remaining terms (the
+// atan2 winding of non-integer g.b and the edge terms).  Equal to
+// e^{i volterra_h} up to rounding; P is a unit complex number.
+__device__ __forceinline__ void volterra_phase(const SegParams *__restrict__ s,
+                                               int nseg, double x, double y,
+                                               double z, double a2,
+                                               double &Pr, double &Pi,
+                                               double &R) {
+    Pr = 1.0, Pi = 0.0, R = 0.0;
+    for (int n = 0; n < nseg; ++n) {
+        const SegParams &p = s[n];
+        const double X = x - p.cx, Y = y - p.cy, Z = z - p.cz;
+        const double eta = X * p.xx + Y * p.xy + Z * p.xz;
+        const double Px = X - eta * p.xx, Py = Y - eta * p.xy,
+                     Pz = Z - eta * p.xz;
+        const double d1 = Px * p.e1x + Py * p.e1y + Pz * p.e1z;
+        const double d2 = Px * p.e2x + Py * p.e2y + Pz * p.e2z;
+        const double r2 = d1 * d1 + d2 * d2;
+        if (p.wind == p.wind) {
+            int m = (int)p.wind;
+            if (m != 0 && r2 > 0.0) {      // atan2(0, 0) = 0: factor 1
+                const double rs = rsqrt(r2);
+                const double ur = d1 * rs;
+                const double ui = m < 0 ? -d2 * rs : d2 * rs;
+                m = m < 0 ? -m : m;
+                double wr = ur, wi = ui;
+                for (int j = 1; j < m; ++j) {
+                    const double tr = wr * ur - wi * ui;
+                    wi = wr * ui + wi * ur;
+                    wr = tr;
+                }
+                const double tr = Pr * wr - Pi * wi;
+                Pi = Pr * wi + Pi * wr;
+                Pr = tr;
+            }
+        } else {
+            R += p.c_atan * atan2(d2, d1);
+        }
+        if (p.edge != 0.0) {
+            const double r2c = fmax(r2, a2);
+            const double inv = 1.0 / r2c;
+            R += (p.k_xy * (d1 * d2) + p.k_dd * (d1 * d1 - d2 * d2)) * inv +
+                 p.k_log * log(r2c);
+        }
+    }
+}
+
 // FP32 segment accumulation (the reference's FP32 H grid: each segment's
 // FP64 value is rounded, then summed in float).
 __device__ __forceinline__ float volterra_h_f32acc(
@@ -111,6 +161,25 @@ __device__ __forceinline__ cx<double> phase_f64(double H) {
     double s, c;
     sincos(H, &s, &c);
     return {c, s};
+}
+
+// P e^{iR} (the parts of volterra_phase) in FP64, and in FP32 after an
+// FP64 reduction of R into [-pi, pi].
+__device__ __forceinline__ cx<double> phase_parts_f64(double Pr, double Pi,
+                                                      double R) {
+    double s, c;
+    sincos(R, &s, &c);
+    return {Pr * c - Pi * s, Pr * s + Pi * c};
+}
+
+__device__ __forceinline__ cx<float> phase_parts_f32(double Pr, double Pi,
+                                                     double R) {
+    const double n = rint(R * (1.0 / kTwoPi));
+    const float r = (float)fma(-n, kTwoPi, R);
+    float s, c;
+    sincosf(r, &s, &c);
+    const float pr = (float)Pr, pi = (float)Pi;
+    return {pr * c - pi * s, pr * s + pi * c};
 }
 
 // Deterministic, segment-independent grid description for kernels.

@@ -81,7 +81,12 @@ int prepare_segments(const double *segs, int nseg, const double *g,
         p.k_dd = -kTwoPi * ge2 * c1 / (4.0 * onu);
         p.k_log = -kTwoPi * ge2 * c1 * (1.0 - 2.0 * nu) / (4.0 * onu);
         p.edge = b_perp > 1e-20 ? 1.0 : 0.0;
-        p.pad_ = 0.0;
+        const double nw = std::nearbyint(p.c_atan);
+        p.wind = std::fabs(p.c_atan - nw) <=
+                             1e-9 * std::fmax(1.0, std::fabs(p.c_atan)) &&
+                         std::fabs(nw) <= 8.0
+                     ? nw
+                     : std::nan("");
     }
     return kept;
 }

@@ -74,11 +74,12 @@ __global__ __launch_bounds__(256) void phase_from_segments_kernel(
     if (x < Nx) {
         const double xp = ((double)x - (double)Nx / 2.0) * dx;
         const double yp = (double)(iy_lo + r - Ny / 2) * dx;
-        const double H = volterra_h(c_segs, nseg, xp, yp, (double)k * dz, a2);
+        double Pr, Pi, R;
+        volterra_phase(c_segs, nseg, xp, yp, (double)k * dz, a2, Pr, Pi, R);
         if constexpr (sizeof(T) == 8)
-            e = phase_f64(H);
+            e = phase_parts_f64(Pr, Pi, R);
         else
-            e = phase_f32(H);
+            e = phase_parts_f32(Pr, Pi, R);
     }
     E[((size_t)r * nsteps + k) * (size_t)(C * kWave) + t] = e;
 }
@@ -101,11 +102,12 @@ __global__ __launch_bounds__(256) void phase_segments_dev_kernel(
     if (x < Nx) {
         const double xp = ((double)x - (double)Nx / 2.0) * dx;
         const double yp = (double)(iy_lo + r - Ny / 2) * dx;
-        const double H = volterra_h(segs, nseg, xp, yp, (double)k * dz, a2);
+        double Pr, Pi, R;
+        volterra_phase(segs, nseg, xp, yp, (double)k * dz, a2, Pr, Pi, R);
         if constexpr (sizeof(T) == 8)
-            e = phase_f64(H);
+            e = phase_parts_f64(Pr, Pi, R);
         else
-            e = phase_f32(H);
+            e = phase_parts_f32(Pr, Pi, R);
     }
     E[((size_t)r * nsteps + k) * (size_t)(C * kWave) + t] = e;
 }
@@ -186,8 +188,9 @@ __global__ __launch_bounds__(256) void bragg_phase_kernel(
     const int k = blockIdx.y, j = blockIdx.z;
     if (ix >= Nx) return;
     const double x = ((double)ix - (double)Nx / 2.0) * dx;
-    const double H = volterra_h(c_segs, nseg, x, c_yvals[j], (double)k * dz, a2);
-    E[((size_t)k * ny + j) * Nx + ix] = phase_f64(H);
+    double Pr, Pi, R;
+    volterra_phase(c_segs, nseg, x, c_yvals[j], (double)k * dz, a2, Pr, Pi, R);
+    E[((size_t)k * ny + j) * Nx + ix] = phase_parts_f64(Pr, Pi, R);
 }
 
 }  // namespace dyf

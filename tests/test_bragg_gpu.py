@@ -61,6 +61,25 @@ def test_phase_march_and_preconditioner_match_oracle(dev, diamond400, g, rng):
         assert _rel(got_w, want_w) < 1e-12
 
 
+def test_phase_table_integer_and_fractional_winding(dev, diamond400, g):
+    """e^{iH} with g.b = +-2 (the winding taken algebraically, no atan2),
+    with g.b = 0.74 (atan2), and with both in one list, against the
+    oracle's exp(iH); the planes include y = 0, where a voxel lies on the
+    threading line (atan2(0, 0) = 0, the core floor a^2)."""
+    cfg = BraggConfig(diamond400, t_crystal=4e-6, dx=0.2e-6, half_x=2e-6)
+    b = diamond400.b_mag
+    whole = (threading_segment("Edge", +1, b, float(g[2]))
+             + threading_segment("Screw", -1, b, float(g[2]), x0=0.6e-6,
+                                 y0=-0.2e-6, azimuth_deg=30.0))
+    frac = threading_segment("Edge", +1, 0.37 * b, float(g[2]), x0=-0.4e-6,
+                             y0=0.2e-6)
+    ys = np.array([-0.2e-6, 0.0, 0.3e-6])
+    for segs in (whole, frac, whole + frac):
+        E = BS.chunk_phase(segs, cfg, diamond400, g, ys)
+        _, Cb, _ = OB.couplings(segs, cfg, diamond400, g, ys)
+        assert _rel(E.get(), Cb / complex(diamond400.sig_hbar)) < 1e-12
+
+
 def test_surface_batch_matches_oracle(dev, diamond400, g):
     cfg = BraggConfig(diamond400, t_crystal=6e-6, dx=0.1e-6, half_x=3e-6)
     sd = float(deviation_from_urad(-6.5, diamond400))
