@@ -76,6 +76,24 @@ class GmresInfo(C.Structure):
                 ("code", C.c_int32)]
 
 
+class PupilArgs(C.Structure):
+    _fields_ = [("Nx", C.c_int32), ("Ny", C.c_int32), ("dx", C.c_double),
+                ("aperture_f", C.c_double), ("ax", C.c_double),
+                ("ay", C.c_double), ("sx", C.c_double), ("sy", C.c_double),
+                ("ell", C.c_int32), ("levels", C.c_int32),
+                ("zero_dc", C.c_int32), ("reserved", C.c_int32)]
+
+
+class FixedOpts(C.Structure):
+    _fields_ = [("tol", C.c_double), ("n_iter", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+class FixedInfo(C.Structure):
+    _fields_ = [("residual", C.c_double), ("iterations", C.c_int32),
+                ("converged", C.c_int32)]
+
+
 _SIGNATURES = {
     "dyf_init": [C.c_int],
     "dyf_device_count": [C.POINTER(C.c_int)],
@@ -162,6 +180,22 @@ _SIGNATURES = {
                         C.POINTER(GmresOpts), vp, vp, vp, C.c_size_t,
                         C.POINTER(GmresInfo), vp],
     "dyf_bragg_exit_wave": [C.POINTER(BraggArgs), C.c_int, vp, vp, vp, vp],
+    "dyf_optics_pupil": [C.POINTER(PupilArgs), C.c_int, vp, vp],
+    "dyf_optics_spectrum": [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp,
+                            vp],
+    "dyf_optics_image_from_spectrum": [vp, C.c_int, C.c_int, vp, C.c_int,
+                                       C.c_int, C.c_int, C.c_int, vp, vp, vp],
+    "dyf_optics_image_bytes": [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                               C.c_int, C.POINTER(C.c_size_t)],
+    "dyf_optics_image": [vp, C.c_int, C.c_int, vp, C.c_int, C.c_int, C.c_int,
+                         C.c_int, vp, vp, vp],
+    "dyf_optics_com_accumulate": [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp,
+                                  vp, vp, vp, vp],
+    "dyf_bragg_fixed_point_bytes": [C.POINTER(BraggArgs), C.c_int,
+                                    C.POINTER(C.c_size_t)],
+    "dyf_bragg_fixed_point": [C.POINTER(BraggArgs), vp, C.c_int, vp, vp,
+                              C.POINTER(FixedOpts), vp, C.c_size_t, vp,
+                              C.POINTER(FixedInfo), vp],
 }
 
 _lib = None

@@ -2,6 +2,7 @@
 
     surface, info = solve_surface_batch(segs, s_dev, cfg, ys, xtal, g)
     wave, info    = solve_bragg_stable(segs, s_dev, cfg, xtal, g)
+    wave, info    = solve_bragg(segs, s_dev, cfg, xtal, g)   # fixed point
 
 Both amplitudes are marched down from the surface with D_g(0) = v
 unknown; the back-face condition D_g(t) = 0 gives A v = -d, A v the
@@ -333,6 +334,43 @@ def solve_surface_batch(seg_list, s_dev, cfg, y_values, xtal, g_vec,
     if scalar:
         return out[0], infos[0]
     return out, infos
+
+
+def solve_bragg(seg_list, s_dev, cfg, xtal, g_vec, xp=None, n_iter=400,
+                tol=1e-11, verbose=False, y_plane=0.0):
+    """Fixed-point (trapezoid round-trip) solve on one y-plane
+    (bragg/solver.py:63-132): D_0 marched forward with D_g frozen, D_g
+    marched back from D_g(t) = 0 with D_0 frozen, until the relative
+    change of D_g(0) is below `tol` or `n_iter` sweeps; uniform unit
+    incidence.  Returns the (Nx,) exit wave D_g(0) e^{-iH(0)} and
+    {"iterations", "residual", "converged"}.  The whole iteration runs in
+    one device launch.  Superset: an array of s_dev returns (n_s, Nx) and
+    a list of info dicts.  `xp` is accepted for compatibility."""
+    scalar = np.ndim(s_dev) == 0
+    s = np.atleast_1d(np.asarray(s_dev, np.float64))
+    E = chunk_phase(seg_list, cfg, xtal, g_vec, [float(y_plane)])
+    a = bragg_args(cfg, xtal, 1)
+    nb = C.c_size_t()
+    N.call("dyf_bragg_fixed_point_bytes", C.byref(a), len(s), C.byref(nb))
+    work = N.DeviceArray.empty((nb.value,), np.uint8)
+    sd = N.DeviceArray.from_host(s)
+    out = N.DeviceArray.empty((len(s), 1, cfg.Nx), np.complex128)
+    opts = N.FixedOpts(tol=float(tol), n_iter=int(n_iter), reserved=0)
+    info = (N.FixedInfo * len(s))()
+    N.call("dyf_bragg_fixed_point", C.byref(a), C.c_void_p(sd.ptr), len(s),
+           C.c_void_p(E.ptr), None, C.byref(opts), C.c_void_p(work.ptr),
+           C.c_size_t(work.nbytes), C.c_void_p(out.ptr), info, None)
+    ew = out.get()[:, 0]
+    infos = [{"iterations": int(info[j].iterations),
+              "residual": float(info[j].residual),
+              "converged": bool(info[j].converged)} for j in range(len(s))]
+    if verbose:
+        for j, d in enumerate(infos):
+            print(f"    s {j}: {d['iterations']} sweeps, residual "
+                  f"{d['residual']:.3e}", flush=True)
+    if scalar:
+        return ew[0], infos[0]
+    return ew, infos
 
 
 def solve_bragg_stable(seg_list, s_dev, cfg, xtal, g_vec, xp=None,

@@ -563,6 +563,152 @@ static int dots(const cd *A, size_t a_sys, size_t a_vec, int kc, const cd *Bv,
     return DYF_OK;
 }
 
+// ---- fixed-point (trapezoid round-trip) solve, bragg/solver.py:63-132 -
+// One workgroup per (system, plane) runs every iteration.  Each thread
+// owns columns x = tid + 256 m of both fields (kept in global memory: the
+// other sweep needs every depth); only the periodic one-column shift
+// crosses threads, through a double-buffered LDS row, so one barrier per
+// depth step suffices.  The next step's e^{iH} and frozen-field values
+// are loaded before the barrier of the current step.
+constexpr int kFpThreads = 256;
+
+__device__ __forceinline__ double block_max(double v, double *red) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    const int w = threadIdx.x >> 6;
+    __syncthreads();                       // red[] free from the last use
+    if ((threadIdx.x & 63) == 0) red[w] = v;
+    __syncthreads();
+    double m = red[0];
+    for (int i = 1; i < kFpThreads / 64; ++i) m = fmax(m, red[i]);
+    return m;
+}
+
+__global__ __launch_bounds__(kFpThreads) void bragg_fixed_point_kernel(
+    BraggK k, const double *__restrict__ s_dev, const cd *__restrict__ E,
+    const cd *__restrict__ inc, int n_iter, double tol, cd *__restrict__ D0,
+    cd *__restrict__ Dg, cd *__restrict__ out, dyf_fixed_info *__restrict__ info) {
+    extern __shared__ cd lds_fp[];
+    __shared__ double red[kFpThreads / 64];
+    const int sys = blockIdx.x / k.nyc, j = blockIdx.x % k.nyc;
+    const int Nx = k.Nx, Nz = k.Nz, tid = threadIdx.x;
+    const cd *Es = E + table_offset(k, sys) + (size_t)j * Nx;
+    const size_t zs = (size_t)k.tpl * Nx;             // e^{iH} depth stride
+    cd *d0 = D0 + (size_t)blockIdx.x * Nz * Nx;
+    cd *dg = Dg + (size_t)blockIdx.x * Nz * Nx;
+    const double alpha = -2.0 * kPi * s_dev[sys];
+    const cd e0 = cexp(k.s0r * k.ds, k.s0i * k.ds);
+    const cd eg = cexp(k.s0r * k.ds, (k.s0i + alpha) * k.ds);
+    const cd sb = {k.sbr, k.sbi}, sh = {k.shr, k.shi};
+    const double hds = 0.5 * k.ds;
+
+    cd a[kMaxPerThread], b[kMaxPerThread], pv[kMaxPerThread];
+#pragma unroll
+    for (int m = 0; m < kMaxPerThread; ++m) {
+        const int x = tid + m * kFpThreads;
+        if (x >= Nx) continue;
+        d0[x] = inc ? inc[x] : cd{1.0, 0.0};
+        for (int z = 0; z < Nz; ++z) dg[(size_t)z * Nx + x] = cd{0.0, 0.0};
+        pv[m] = cd{0.0, 0.0};
+    }
+    double res = __builtin_nan("");
+    int iters = n_iter, conv = 0, par = 0;
+    for (int it = 0; it < n_iter; ++it) {
+        // forward: D0[z+1] = e0 roll(D0[z] + hds Cb[z] Dg[z]) + hds Cb[z+1] Dg[z+1]
+        cd cur[kMaxPerThread], en[kMaxPerThread], gn[kMaxPerThread];
+#pragma unroll
+        for (int m = 0; m < kMaxPerThread; ++m) {
+            const int x = tid + m * kFpThreads;
+            if (x >= Nx) continue;
+            cur[m] = d0[x];
+            en[m] = Es[x];
+            gn[m] = dg[x];
+        }
+        for (int z = 0; z < Nz - 1; ++z) {
+            cd *buf = lds_fp + par * Nx;
+            par ^= 1;
+#pragma unroll
+            for (int m = 0; m < kMaxPerThread; ++m) {
+                const int x = tid + m * kFpThreads;
+                if (x >= Nx) continue;
+                const cd cbg = cmul(cmul(sb, en[m]), gn[m]);
+                buf[x] = cmul(e0, cadd(cur[m], cscale(cbg, hds)));
+                en[m] = Es[(size_t)(z + 1) * zs + x];
+                gn[m] = dg[(size_t)(z + 1) * Nx + x];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < kMaxPerThread; ++m) {
+                const int x = tid + m * kFpThreads;
+                if (x >= Nx) continue;
+                const cd cbg = cmul(cmul(sb, en[m]), gn[m]);
+                cur[m] = cadd(buf[x > 0 ? x - 1 : Nx - 1], cscale(cbg, hds));
+                d0[(size_t)(z + 1) * Nx + x] = cur[m];
+            }
+        }
+        // backward: Dg[z] = eg roll(Dg[z+1] + hds Ch[z+1] D0[z+1]) + hds Ch[z] D0[z]
+        cd nx[kMaxPerThread];
+#pragma unroll
+        for (int m = 0; m < kMaxPerThread; ++m) {
+            const int x = tid + m * kFpThreads;
+            if (x >= Nx) continue;
+            nx[m] = cd{0.0, 0.0};
+            dg[(size_t)(Nz - 1) * Nx + x] = nx[m];
+            en[m] = Es[(size_t)(Nz - 1) * zs + x];
+            gn[m] = cur[m];                          // D0[Nz-1]
+        }
+        for (int z = Nz - 2; z >= 0; --z) {
+            cd *buf = lds_fp + par * Nx;
+            par ^= 1;
+#pragma unroll
+            for (int m = 0; m < kMaxPerThread; ++m) {
+                const int x = tid + m * kFpThreads;
+                if (x >= Nx) continue;
+                const cd chp = cmul(cconjmul(en[m], sh), gn[m]);
+                buf[x] = cmul(eg, cadd(nx[m], cscale(chp, hds)));
+                en[m] = Es[(size_t)z * zs + x];
+                gn[m] = d0[(size_t)z * Nx + x];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < kMaxPerThread; ++m) {
+                const int x = tid + m * kFpThreads;
+                if (x >= Nx) continue;
+                const cd chp = cmul(cconjmul(en[m], sh), gn[m]);
+                nx[m] = cadd(buf[x > 0 ? x - 1 : Nx - 1], cscale(chp, hds));
+                dg[(size_t)z * Nx + x] = nx[m];
+            }
+        }
+        // convergence on the surface value Dg[0]
+        double dmax = 0.0, amax = 0.0;
+#pragma unroll
+        for (int m = 0; m < kMaxPerThread; ++m) {
+            const int x = tid + m * kFpThreads;
+            if (x >= Nx) continue;
+            const double dr = nx[m].r - pv[m].r, di = nx[m].i - pv[m].i;
+            dmax = fmax(dmax, sqrt(dr * dr + di * di));
+            amax = fmax(amax, sqrt(nx[m].r * nx[m].r + nx[m].i * nx[m].i));
+            pv[m] = nx[m];
+        }
+        dmax = block_max(dmax, red);
+        amax = block_max(amax, red);
+        if (it > 0) {
+            res = dmax / (amax + 1e-300);
+            if (res < tol) {
+                conv = 1, iters = it + 1;
+                break;
+            }
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < kMaxPerThread; ++m) {
+        const int x = tid + m * kFpThreads;
+        if (x >= Nx) continue;
+        out[(size_t)blockIdx.x * Nx + x] = cconjmul(Es[x], pv[m]);
+    }
+    if (tid == 0) info[blockIdx.x] = dyf_fixed_info{res, iters, conv, };
+}
+
 }  // namespace dyf
 
 using namespace dyf;
@@ -629,6 +775,51 @@ extern "C" int dyf_bragg_exit_wave(const dyf_bragg_args *a, int nsys,
     exit_wave_kernel<<<blocks, 256, 0, (hipStream_t)stream>>>(
         (const cd *)v, (const cd *)E, make_k(a, nsys), n, (cd *)out);
     DYF_LAUNCHED("exit_wave_kernel");
+    return DYF_OK;
+}
+
+extern "C" int dyf_bragg_fixed_point_bytes(const dyf_bragg_args *a, int nsys,
+                                           size_t *bytes) {
+    int rc = check_bragg(a, nsys);
+    if (rc) return rc;
+    if (!bytes) return fail(DYF_EINVAL, "bytes is NULL");
+    const size_t n = (size_t)nsys * a->nyc;
+    *bytes = 2 * n * a->Nz * a->Nx * sizeof(cd) + n * sizeof(dyf_fixed_info);
+    return DYF_OK;
+}
+
+extern "C" int dyf_bragg_fixed_point(const dyf_bragg_args *a,
+                                     const double *s_dev, int nsys,
+                                     const void *E, const void *incident,
+                                     const dyf_fixed_opts *opt, void *work,
+                                     size_t work_bytes, void *out,
+                                     dyf_fixed_info *info, void *stream_) {
+    int rc = check_bragg(a, nsys);
+    if (rc) return rc;
+    if (!opt || opt->n_iter < 1 || !(opt->tol >= 0.0))
+        return fail(DYF_EINVAL, "bad fixed-point options");
+    if (!s_dev || !E || !work || !out || !info)
+        return fail(DYF_EINVAL, "NULL argument to dyf_bragg_fixed_point");
+    size_t need = 0;
+    dyf_bragg_fixed_point_bytes(a, nsys, &need);
+    if (work_bytes < need)
+        return fail(DYF_EINVAL, "workspace of %zu bytes, need %zu", work_bytes,
+                    need);
+    hipStream_t stream = (hipStream_t)stream_;
+    const BraggK k = make_k(a, nsys);
+    const size_t n = (size_t)nsys * a->nyc, field = n * a->Nz * a->Nx;
+    cd *D0 = (cd *)work, *Dg = D0 + field;
+    dyf_fixed_info *idev = (dyf_fixed_info *)(Dg + field);
+    // two rows of at most 2048 columns: within the default 64 KB of LDS
+    const size_t lds = (size_t)2 * a->Nx * sizeof(cd);
+    hipLaunchKernelGGL(bragg_fixed_point_kernel, dim3((unsigned)n),
+                       dim3(kFpThreads), lds, stream, k, s_dev, (const cd *)E,
+                       (const cd *)incident, opt->n_iter, opt->tol, D0, Dg,
+                       (cd *)out, idev);
+    DYF_LAUNCHED("bragg_fixed_point_kernel");
+    DYF_HIP(hipMemcpyAsync(info, idev, n * sizeof(dyf_fixed_info),
+                           hipMemcpyDeviceToHost, stream));
+    DYF_HIP(hipStreamSynchronize(stream));
     return DYF_OK;
 }
 

@@ -250,6 +250,87 @@ int dyf_bragg_solve(const dyf_bragg_args *a, const double *s_dev_dev,
 int dyf_bragg_exit_wave(const dyf_bragg_args *a, int nsys, const void *E_dev,
                         const void *v_dev, void *out_dev, void *stream);
 
+/* The fixed-point (trapezoid round-trip) solve, solve_bragg
+ * (bragg/solver.py:63-132): forward march of D0 with Dg frozen, backward
+ * march of Dg with D0 frozen, until max|Dg(0) - previous| / max|Dg(0)| <
+ * tol or n_iter sweeps.  One workgroup per (system, plane) runs every
+ * iteration in one launch.  incident_dev: D0 at z = 0 (Nx complex), NULL
+ * for the reference's uniform 1.  work_dev: dyf_bragg_fixed_point_bytes
+ * (the D0 and Dg fields of every system).  out_dev (nsys, nyc, Nx) gets
+ * the exit wave Dg(0) e^{-iH(0)}; info (host, nsys * nyc entries) gets
+ * each plane's iterations, last residual and convergence.  Synchronous. */
+typedef struct {
+    double tol;
+    int32_t n_iter;
+    int32_t reserved;
+} dyf_fixed_opts;
+
+typedef struct {
+    double residual;      /* NaN when only one sweep pair ran */
+    int32_t iterations;
+    int32_t converged;
+} dyf_fixed_info;
+
+int dyf_bragg_fixed_point_bytes(const dyf_bragg_args *a, int nsys,
+                                size_t *bytes);
+int dyf_bragg_fixed_point(const dyf_bragg_args *a, const double *s_dev_dev,
+                          int nsys, const void *E_dev,
+                          const void *incident_dev, const dyf_fixed_opts *opt,
+                          void *work_dev, size_t work_bytes, void *out_dev,
+                          dyf_fixed_info *info, void *stream);
+
+/* ---- Exit wave -> image (optics.py:55-207, analysis/rocking.py:37-83) */
+/* Fields, spectra, pupils and images are (Ny, Nx) row-major, complex
+ * interleaved, batches contiguous; `prec` DYF_FP64 = complex128/float64,
+ * otherwise complex64/float32.  Pupils are on the fftshifted frequency
+ * grid, as the reference's are.                                         */
+typedef struct {
+    int32_t Nx, Ny;
+    double dx;            /* object pixel (m)                             */
+    double aperture_f;    /* NA / lambda: the disc radius (1/m)           */
+    double ax, ay;        /* aperture centre offset (1/m)                 */
+    double sx, sy;        /* spiral singularity offset (1/m)              */
+    int32_t ell;          /* spiral charge                                */
+    int32_t levels;       /* phase steps of a quantised plate (0: none)   */
+    int32_t zero_dc;      /* 1: zero the bin (Ny/2, Nx/2) (ell != 0 only) */
+    int32_t reserved;
+} dyf_pupil_args;
+
+/* Optics.pupil (optics.py:67-113): the NA disc times e^{i ell phi}. */
+int dyf_optics_pupil(const dyf_pupil_args *a, int prec, void *out_dev,
+                     void *stream);
+/* object_spectrum (optics.py:172-182): fftshift(fft2(fftshift(f))) of n
+ * fields; work_dev holds n planes of the field precision.              */
+int dyf_optics_spectrum(const void *field_dev, int n, int Nx, int Ny,
+                        int prec, void *work_dev, void *spec_dev,
+                        void *stream);
+/* image_from_spectrum (optics.py:185-194): |fftshift(ifft2(ifftshift(S P)))|^2
+ * for n spectra, one pupil shared (pupil_per_item = 0) or one per item;
+ * work_dev holds 2 n planes of the product precision (FP64 if either
+ * input is).                                                            */
+int dyf_optics_image_from_spectrum(const void *spec_dev, int n,
+                                   int spec_prec, const void *pupil_dev,
+                                   int pupil_per_item, int pupil_prec,
+                                   int Nx, int Ny, void *work_dev,
+                                   void *out_dev, void *stream);
+/* Optics.image / images, apply_pupils, image_intensity (optics.py:144-
+ * 207): n fields through npup pupils, out (n, npup, Ny, Nx) real, one
+ * forward transform per field and no shift copies.  work_dev:
+ * dyf_optics_image_bytes.                                               */
+int dyf_optics_image_bytes(int n, int npup, int field_prec, int pupil_prec,
+                           int Nx, int Ny, size_t *bytes);
+int dyf_optics_image(const void *field_dev, int n, int field_prec,
+                     const void *pupils_dev, int npup, int pupil_prec,
+                     int Nx, int Ny, void *work_dev, void *out_dev,
+                     void *stream);
+/* com_maps' accumulation (analysis/rocking.py:66-75): sum_wI += sum_i
+ * w[i] I[i], sum_I += sum_i I[i] (FP64, per pixel, in batch order), and
+ * image_sums[i] = sum over I[i] (FP64; NULL to skip).                   */
+int dyf_optics_com_accumulate(const void *images_dev, int n, int prec,
+                              int Nx, int Ny, const double *weights_dev,
+                              double *sum_wI_dev, double *sum_I_dev,
+                              double *image_sums_dev, void *stream);
+
 /* ---- Learned warm start: BF16-on-MFMA network kernels --------------- */
 /* No counterpart in the reference (north-star addition).  Tensors are
  * NHWC; bf16 is stored as uint16 (bfloat16 bits).  A layer reads a channel

@@ -9,6 +9,7 @@ Restates the reference's bragg/solver.py:
                        1e-8 max|T22| by a real value            (163-168)
     solve_bragg_stable one plane, SciPy GMRES restart 40        (135-179)
     solve_surface_batch chunked planes, retry/grow/split policy (182-379)
+    fixed_point        the trapezoid round-trip fixed point      (63-132)
 
 with Ch = sigma_h e^{-iH}, Cb = sigma_hbar e^{+iH}, e0 = e^{s0 ds},
 eg = e^{(s0 + i alpha) ds}, alpha = -2 pi s_dev, and the exit wave
@@ -162,3 +163,71 @@ def dense_shooting_solution(seg_list, s_dev, cfg, y_plane, xtal, g_vec):
         A[:, j] = march_down(zero, e, Ch, Cb, e0, eg, ds, cfg.Nz)[0]
     v = np.linalg.solve(A, -d)
     return v * np.exp(-1j * H0[0]), A, d
+
+
+def fixed_point(seg_list, s_dev, cfg, xtal, g_vec, n_iter=400, tol=1e-11,
+                y_plane=0.0):
+    """The reference's solve_bragg (bragg/solver.py:63-132): trapezoid
+    forward march of D0 with Dg frozen, backward march of Dg from
+    Dg[Nz-1] = 0 with D0 frozen, uniform unit incidence, stop when
+    max|Dg[0] - previous| / (max|Dg[0]| + 1e-300) < tol.  Returns the
+    exit wave Dg[0] e^{-iH[0]} and (iterations, residual, converged)."""
+    Ch, Cb, H = couplings(seg_list, cfg, xtal, g_vec, [float(y_plane)])
+    Ch, Cb, H0 = Ch[:, 0], Cb[:, 0], H[0]
+    e0, eg = step_constants(s_dev, complex(xtal.sig0), cfg.ds)
+    ds, Nz, Nx = cfg.ds, cfg.Nz, cfg.Nx
+    D0 = np.zeros((Nz, Nx), complex)
+    Dg = np.zeros((Nz, Nx), complex)
+    D0[0] = 1.0
+    prev, res, its, conv = None, np.nan, n_iter, False
+    for it in range(n_iter):
+        for k in range(Nz - 1):
+            D0[k + 1] = (e0 * np.roll(D0[k], 1)
+                         + 0.5 * ds * (e0 * np.roll(Cb[k] * Dg[k], 1)
+                                       + Cb[k + 1] * Dg[k + 1]))
+        Dg[Nz - 1] = 0.0
+        for k in range(Nz - 2, -1, -1):
+            Dg[k] = (eg * np.roll(Dg[k + 1], 1)
+                     + 0.5 * ds * (eg * np.roll(Ch[k + 1] * D0[k + 1], 1)
+                                   + Ch[k] * D0[k]))
+        cur = Dg[0].copy()
+        if prev is not None:
+            res = np.abs(cur - prev).max() / (np.abs(cur).max() + 1e-300)
+            if res < tol:
+                its, conv = it + 1, True
+                break
+        prev = cur
+    return Dg[0] * np.exp(-1j * H0), (its, float(res), conv)
+
+
+def fixed_point_discrete_perfect(s_dev, cfg, xtal):
+    """The fixed point's limit for H = 0 and uniform incidence, by a dense
+    solve of the discrete two-point problem it iterates on: with no x
+    dependence the rolls drop out and D0[k+1] = e0 D0[k] + ds/2 (e0 sb
+    Dg[k] + sb Dg[k+1]), Dg[k] = eg Dg[k+1] + ds/2 (eg sh D0[k+1] + sh
+    D0[k]), D0[0] = 1, Dg[Nz-1] = 0: 2 Nz linear equations.  Returns
+    Dg[0]."""
+    e0, eg = step_constants(s_dev, complex(xtal.sig0), cfg.ds)
+    sh, sb, h = complex(xtal.sig_h), complex(xtal.sig_hbar), 0.5 * cfg.ds
+    Nz = cfg.Nz
+    A = np.zeros((2 * Nz, 2 * Nz), complex)
+    b = np.zeros(2 * Nz, complex)
+    d0 = lambda k: k                          # unknown indices
+    dg = lambda k: Nz + k
+    A[0, d0(0)] = 1.0
+    b[0] = 1.0
+    for k in range(Nz - 1):                   # rows 1 .. Nz-1
+        r = 1 + k
+        A[r, d0(k + 1)] = 1.0
+        A[r, d0(k)] = -e0
+        A[r, dg(k)] = -h * e0 * sb
+        A[r, dg(k + 1)] = -h * sb
+    A[Nz, dg(Nz - 1)] = 1.0                   # back face
+    for k in range(Nz - 1):                   # rows Nz+1 .. 2Nz-1
+        r = Nz + 1 + k
+        A[r, dg(k)] = 1.0
+        A[r, dg(k + 1)] = -eg
+        A[r, d0(k + 1)] = -h * eg * sh
+        A[r, d0(k)] = -h * sh
+    x = np.linalg.solve(A, b)
+    return x[dg(0)]

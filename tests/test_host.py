@@ -131,3 +131,16 @@ def test_grid_key(diamond400):
     assert a.key() != a.replace(grid_pad=14).key()
     assert a.key() == a.replace().key()
     assert "Ny" not in a.key()
+
+
+def test_planewave_rocking_is_the_closed_form(diamond400):
+    from dyfox_amd.rocking import planewave_rocking
+    from oracle import closed_forms as CF
+    s = deviation_from_urad(np.linspace(-30, 30, 13), diamond400)
+    for absorb in (True, False):
+        want = CF.planewave_rocking(
+            s, complex(diamond400.sig0), complex(diamond400.sig_h),
+            complex(diamond400.sig_hbar), diamond400.cos_tB, 40e-6,
+            absorption=absorb)
+        got = planewave_rocking(s, diamond400, 40e-6, absorption=absorb)
+        assert np.allclose(got, want, rtol=1e-12, atol=0)

@@ -42,3 +42,6 @@ def test_structs_match_the_header_layout():
     assert ctypes.sizeof(_native.LineParams) == 12 * 8
     assert ctypes.sizeof(_native.VolumeDesc) == 24
     assert ctypes.sizeof(_native.BraggArgs) == 7 * 8 + 6 * 4
+    assert ctypes.sizeof(_native.PupilArgs) == 2 * 4 + 6 * 8 + 4 * 4
+    assert ctypes.sizeof(_native.FixedOpts) == 16
+    assert ctypes.sizeof(_native.FixedInfo) == 16

@@ -138,3 +138,33 @@ def test_oracle_zero_b_is_perfect(diamond400, g):
     a, _ = OB.solve_surface_batch(dead, sd, cfg, np.zeros(1), diamond400, g)
     b, _ = OB.solve_surface_batch([], sd, cfg, np.zeros(1), diamond400, g)
     assert np.abs(a - b).max() < 1e-12
+
+
+def test_fixed_point_perfect_is_the_dense_discrete_solution(diamond400, g):
+    """H = 0, uniform incidence: the round trip converges to the solution
+    of its own discrete two-point problem, solved densely."""
+    cfg = BraggConfig(diamond400, t_crystal=4e-6, dx=0.1e-6, half_x=3e-6)
+    for urad in (-6.5, 0.0, 9.0):
+        sd = float(deviation_from_urad(urad, diamond400))
+        ew, (its, res, conv) = OB.fixed_point([], sd, cfg, diamond400, g,
+                                             n_iter=800, tol=1e-13)
+        want = OB.fixed_point_discrete_perfect(sd, cfg, diamond400)
+        assert conv and its > 1 and res < 1e-13
+        assert np.abs(ew - want).max() <= 1e-12 * abs(want)
+
+
+def test_fixed_point_matches_the_riccati_reference(diamond400, g):
+    """bragg tests: the perfect crystal on the grid against the finite-
+    thickness closed form (test_bragg.py:113-124)."""
+    cfg = BraggConfig(diamond400, t_crystal=8e-6, dx=0.1e-6, half_x=4e-6)
+    for urad in (-6.5, -3.0, 1.0):
+        sd = float(deviation_from_urad(urad, diamond400))
+        ew, (_, _, conv) = OB.fixed_point([], sd, cfg, diamond400, g,
+                                          n_iter=600, tol=1e-13)
+        ref = abs(CF.reflectivity_finite(
+            np.array([sd]), cfg.t_crystal, complex(diamond400.sig0),
+            complex(diamond400.sig_h), complex(diamond400.sig_hbar),
+            diamond400.sin_tB)[0])
+        assert conv
+        assert abs(abs(ew[cfg.Nx // 2]) - ref) <= 2e-3 * ref
+        assert np.abs(ew).max() <= 1.0 + 1e-9          # no amplification
