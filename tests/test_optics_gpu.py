@@ -141,3 +141,32 @@ def test_com_maps_match_the_oracle_sums(dev, diamond400, small_grid, lens,
         assert _rel(weight[ell], b) < tol and _rel(rock[ell], t) < tol
         assert _rel(com[ell] * b, a) < tol
 
+
+
+def test_incoherent_image_matches_the_oracle_sum(dev, diamond400, small_grid,
+                                                 lens, rng):
+    from dyfox_amd.illumination import (convergence_bandwidth_nodes,
+                                        incoherent_image)
+    segs = random_segments(rng, diamond400, small_grid, 2)
+    s, d, w, _ = convergence_bandwidth_nodes(diamond400, 20e-6, 1e-4, 3,
+                                             60e-6, 3, 3e-4)
+    Ny, Nx = small_grid.shape
+    stack = to_numpy(solve_dislocation(segs, s, small_grid,
+                                       diamond400)).astype(complex)
+    spec = OO.object_spectrum(stack)
+    dfx = diamond400.sin_2tB * d / diamond400.lam
+    for ell in (0, 2):
+        got = to_numpy(incoherent_image(segs, small_grid, diamond400, lens,
+                                        s, w, d, ell=ell, batch=4))
+        want = sum(w[n] * OO.image_from_spectrum(
+            spec[n], OO.pupil(Nx, Ny, small_grid.dx, ell, lens.aperture_f,
+                              aperture_f=(dfx[n], 0.0),
+                              singularity_f=(dfx[n], 0.0), dc_zero="never"))
+            for n in range(len(s)))
+        assert _rel(got, want) < 1e-12
+    plain = to_numpy(incoherent_image(segs, small_grid, diamond400, lens, s,
+                                      w, ell=2, batch=5))
+    pup = OO.pupil(Nx, Ny, small_grid.dx, 2, lens.aperture_f)
+    want = sum(w[n] * OO.image_from_spectrum(spec[n], pup)
+               for n in range(len(s)))
+    assert _rel(plain, want) < 1e-12
