@@ -144,6 +144,70 @@ def time_ensembles():
     return out
 
 
+def time_imaging(N, stack, grid, xtal, repeats=5):
+    """The exit-wave stack of the rocking scan (64 x 256 x 1056 complex)
+    imaged through the open and the charge-2 pupil (Optics.image: one
+    batched forward 2-D FFT, pupil multiply, one batched inverse, |.|^2),
+    timed with events; the oracle's NumPy imaging of 4 of the images on
+    the host beside it."""
+    from dyfox_amd.optics import Optics
+    from oracle import optics as OO
+    lens = Optics(1e-4, 0.274, xtal)
+    Ny, Nx = stack.shape[-2:]
+    for ell in (0, 2):
+        lens.image(stack, grid.dx, ell=ell)                # plans, pupils
+    N.synchronize()
+    ev = [N.Event() for _ in range(2)]
+    ev[0].record()
+    for _ in range(repeats):
+        for ell in (0, 2):
+            img = lens.image(stack, grid.dx, ell=ell)
+    ev[1].record()
+    N.synchronize()
+    ms = ev[0].elapsed_ms(ev[1]) / repeats
+    n_img = 2 * stack.shape[0]
+    # bytes touched per image: field read, spectrum write + read (forward),
+    # product write + read (inverse), image write: 5 complex + 1 real planes
+    plane = Ny * Nx
+    nbytes = n_img * plane * (5 * 16 + 8)
+    host = stack[:4].get().astype(complex)
+    pup = OO.pupil(Nx, Ny, grid.dx, 2, lens.aperture_f)
+    t0 = time.perf_counter()
+    OO.image_from_spectrum(OO.object_spectrum(host), pup)
+    cpu = 4 / (time.perf_counter() - t0)
+    del img
+    return {"images_per_s": n_img / (ms * 1e-3), "ms_per_scan_2_pupils": ms,
+            "grid": f"{stack.shape[0]} x {Ny} x {Nx}",
+            "effective_GB_per_s": nbytes / (ms * 1e-3) / 1e9,
+            "cpu_oracle_images_per_s": cpu}
+
+
+def time_fixed_point(N, xtal):
+    """The reference's fixed-point Bragg solve (bragg/solver.py:63-132)
+    on its own test grid (t = 8 um, dx = 0.1 um, Nz 180, Nx 832), 41
+    angles in one launch, against the NumPy oracle for one angle."""
+    from dyfox_amd.bragg import BraggConfig, bragg_g_vector
+    from dyfox_amd.bragg.solver import solve_bragg
+    from dyfox_amd.units import deviation_from_urad
+    from oracle import bragg as OB
+    g = bragg_g_vector(xtal)
+    cfg = BraggConfig(xtal, t_crystal=8e-6, dx=0.1e-6, half_x=4e-6)
+    s = deviation_from_urad(np.linspace(-60, 60, 41), xtal)
+    solve_bragg([], s, cfg, xtal, g, n_iter=600, tol=1e-13)
+    t0 = time.perf_counter()
+    _, infos = solve_bragg([], s, cfg, xtal, g, n_iter=600, tol=1e-13)
+    dt = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    OB.fixed_point([], float(s[20]), cfg, xtal, g, n_iter=600, tol=1e-13)
+    cpu = time.perf_counter() - t0
+    its = [i["iterations"] for i in infos]
+    return {"solves_per_s": len(s) / dt, "ms_per_batch": dt * 1e3,
+            "sweeps": [min(its), max(its)],
+            "all_converged": all(i["converged"] for i in infos),
+            "grid": f"Nz={cfg.Nz} Nx={cfg.Nx}",
+            "cpu_oracle_solves_per_s": 1.0 / cpu}
+
+
 def cpu_baseline(segs, s, grid, xtal, rows=48, n_s=8):
     """The NumPy FP64 oracle (Volterra H + direct-form march) on a bounded
     sample: `rows` marched rows x `n_s` angles of the same workload."""
@@ -262,6 +326,9 @@ def main():
           f"{mmx:.2f})", file=sys.stderr, flush=True)
     rep = error_report(outmx.get(), out64.get())
     worst = {k: max(r[k] for r in rep) for k in rep[0]}
+    imaging = time_imaging(N, out64, grid, xtal)
+    print(f"[bench] imaging {imaging['images_per_s']:.0f} images/s",
+          file=sys.stderr, flush=True)
     del out64, outmx
 
     g2, seg2, s2 = laue_worked_example(xtal)
@@ -311,6 +378,8 @@ def main():
                                 "phase_ms": p2, "march_ms": m2,
                                 "pixel_steps": work2},
         "bragg_cfg1_cold": bragg,
+        "bragg_fixed_point": time_fixed_point(N, xtal),
+        "imaging_cfg3": imaging,
         "ensemble_cfg4": None if args.no_ensemble else time_ensembles(),
         "bragg_warm_start_cfg5": warm_start_report(),
         "roofline": roof,
