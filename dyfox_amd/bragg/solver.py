@@ -234,11 +234,10 @@ def solve_surface_batch(seg_list, s_dev, cfg, y_values, xtal, g_vec,
         for _ in range(tries):
             v, info = gmres_batch(cfg, xtal, s_all[pending], E, inc, tol=tol,
                                   restart=r, maxiter=maxiter, x0=x0)
-            vh = v.get()
             for j, s in enumerate(pending):
                 res[s] = info[j][0]
                 cnt[s] += info[j][1]
-                V[s] = vh[j]
+                V[s] = v[j]                     # device view
             retry = [j for j, s in enumerate(pending)
                      if res[s] > accept and r < cap]
             if not retry:
@@ -250,15 +249,19 @@ def solve_surface_batch(seg_list, s_dev, cfg, y_values, xtal, g_vec,
                           f"{res[pending[j]]:.1e} above {accept:.0e}, "
                           f"retrying with restart {min(r * grow, cap)}",
                           flush=True)
-            x0 = N.DeviceArray.from_host(
-                np.ascontiguousarray(vh[retry]))
+            x0 = N.DeviceArray.empty((len(retry), nyc, cfg.Nx),
+                                     np.complex128)
+            for i, j in enumerate(retry):
+                x0[i].copy_from(v[j])
             pending = [pending[j] for j in retry]
             r = min(r * grow, cap)
         split = [s for s in sys_idx
                  if res[s] > accept and nyc > 1 and depth < max_depth]
         keep = [s for s in sys_idx if s not in split]
         if keep:
-            vk = N.DeviceArray.from_host(np.stack([V[s] for s in keep]))
+            vk = N.DeviceArray.empty((len(keep), nyc, cfg.Nx), np.complex128)
+            for j, s in enumerate(keep):
+                vk[j].copy_from(V[s])
             ew = exit_wave(cfg, xtal, E, vk)
             for j, s in enumerate(keep):
                 st = stats[s]
@@ -285,16 +288,20 @@ def solve_surface_batch(seg_list, s_dev, cfg, y_values, xtal, g_vec,
         pending = list(range(n_s))
         x0 = None if warm is None else N.DeviceArray.from_host(
             np.ascontiguousarray(warm[:, j0:j1]))
-        V = np.zeros((n_s, nyc, cfg.Nx), np.complex128)
+        # the solutions stay on the device: only the exit waves come back
+        V = None
         res = np.zeros((n_s, nyc))
         cnt = np.zeros((n_s, nyc), np.int64)
         for _ in range(tries):
             v, info = gmres_batch(cfg, xtal, s_all[pending], E, inc, tol=tol,
                                   restart=r, maxiter=maxiter, x0=x0,
                                   planes=True)
-            vh = v.get()
+            if V is None:
+                V = v
+            else:
+                for j, s in enumerate(pending):
+                    V[s].copy_from(v[j])
             for j, s in enumerate(pending):
-                V[s] = vh[j]
                 for p in range(nyc):
                     rs, it, _ = info[j * nyc + p]
                     res[s, p] = rs
@@ -304,10 +311,12 @@ def solve_surface_batch(seg_list, s_dev, cfg, y_values, xtal, g_vec,
                 break
             for s in retry:
                 stats[s]["retries"] += 1
-            x0 = N.DeviceArray.from_host(np.ascontiguousarray(V[retry]))
+            x0 = N.DeviceArray.empty((len(retry), nyc, cfg.Nx), np.complex128)
+            for j, s in enumerate(retry):
+                x0[j].copy_from(V[s])
             pending = retry
             r = min(r * grow, cap)
-        ew = exit_wave(cfg, xtal, E, N.DeviceArray.from_host(V))
+        ew = exit_wave(cfg, xtal, E, V)
         for s in range(n_s):
             st = stats[s]
             st["unconverged"] += int((res[s] > accept).sum())

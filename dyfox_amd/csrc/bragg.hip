@@ -81,6 +81,12 @@ static BraggK make_k(const dyf_bragg_args *a, int nsys) {
 constexpr int kMarchThreads = 256;
 constexpr int kMaxPerThread = 8;          // Nx <= 2048
 
+// M = ceil(Nx / 256) columns per thread.  The loop is branch-free: loads
+// use a clamped index and out-of-range columns write to a spare LDS slot
+// (index Nx), so the compiler can keep the next step's e^{iH} prefetch in
+// flight across the step (with guarded loads it waited for every load
+// before each column).
+template <int M>
 __global__ __launch_bounds__(kMarchThreads) void bragg_march_kernel(
     BraggK k, const double *__restrict__ s_dev, const cd *__restrict__ E,
     const cd *__restrict__ inc, const cd *__restrict__ vin, size_t vin_stride,
@@ -90,13 +96,15 @@ __global__ __launch_bounds__(kMarchThreads) void bragg_march_kernel(
     const int s = blockIdx.x / k.nyc, j = blockIdx.x % k.nyc;
     if (active && !active[s]) return;
     E += table_offset(k, s);
-    const int Nx = k.Nx, tid = threadIdx.x;
-    cd *D0[2] = {lds_b, lds_b + 2 * Nx};
-    cd *Dg[2] = {lds_b + Nx, lds_b + 3 * Nx};
+    const int Nx = k.Nx, tid = threadIdx.x, P = Nx + 1;
+    // LDS: [D0 | Dg] x 2 buffers of P = Nx + 1 entries, addressed by
+    // integer offsets into lds_b (selecting among pointers kept in an
+    // array loses the LDS address space and turns every access into a
+    // flat one, which then waits for outstanding global loads too)
     for (int x = tid; x < Nx; x += kMarchThreads) {
-        D0[0][x] = inc ? inc[x] : cd{0.0, 0.0};
-        Dg[0][x] = vin ? vin[(size_t)s * vin_stride + (size_t)j * Nx + x]
-                       : cd{0.0, 0.0};
+        lds_b[x] = inc ? inc[x] : cd{0.0, 0.0};
+        lds_b[P + x] = vin ? vin[(size_t)s * vin_stride + (size_t)j * Nx + x]
+                           : cd{0.0, 0.0};
     }
     const double alpha = -2.0 * kPi * s_dev[s];
     const cd e0 = cexp(k.s0r * k.ds, k.s0i * k.ds);
@@ -104,65 +112,85 @@ __global__ __launch_bounds__(kMarchThreads) void bragg_march_kernel(
     const cd ieg = cdiv(cd{1.0, 0.0}, eg);
     const cd A = {k.ds * k.sbr, k.ds * k.sbi};   // ds sigma_hbar
     const cd B = {k.ds * k.shr, k.ds * k.shi};   // ds sigma_h
+    // per column m: the source column (clamped for loads and LDS reads)
+    // and the destinations of the shifted fields (slot Nx when unused)
+    int xs[M], to0[M], tog[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const int x = tid + m * kMarchThreads;
+        const bool ok = x < Nx;
+        xs[m] = ok ? x : Nx - 1;
+        to0[m] = ok ? (x + 1 < Nx ? x + 1 : 0) : Nx;
+        tog[m] = ok ? (x > 0 ? x - 1 : Nx - 1) : Nx;
+    }
     __syncthreads();
     int cur = 0;
     const size_t plane = (size_t)k.tpl * Nx;
-    cd en[kMaxPerThread];
+    const cd *Ej = E + (size_t)j * Nx;
+    cd en[M];
 #pragma unroll
-    for (int m = 0; m < kMaxPerThread; ++m) {
-        const int x = tid + m * kMarchThreads;
-        if (x < Nx) en[m] = E[(size_t)j * Nx + x];
-    }
+    for (int m = 0; m < M; ++m) en[m] = Ej[xs[m]];
     for (int step = 0; step < k.Nz - 1; ++step) {
-        cd ec[kMaxPerThread];
+        cd ec[M];
 #pragma unroll
-        for (int m = 0; m < kMaxPerThread; ++m) ec[m] = en[m];
-        if (step + 1 < k.Nz - 1) {
-            const cd *En = E + (size_t)(step + 1) * plane + (size_t)j * Nx;
+        for (int m = 0; m < M; ++m) ec[m] = en[m];
+        const int sn = step + 1 < k.Nz - 1 ? step + 1 : step;
+        const cd *En = Ej + (size_t)sn * plane;
 #pragma unroll
-            for (int m = 0; m < kMaxPerThread; ++m) {
-                const int x = tid + m * kMarchThreads;
-                if (x < Nx) en[m] = En[x];
-            }
-        }
-        const cd *d0 = D0[cur], *dg = Dg[cur];
-        cd *n0 = D0[1 - cur], *ng = Dg[1 - cur];
+        for (int m = 0; m < M; ++m) en[m] = En[xs[m]];
+        const int o0 = cur * 2 * P, n0 = (1 - cur) * 2 * P;
 #pragma unroll
-        for (int m = 0; m < kMaxPerThread; ++m) {
-            const int x = tid + m * kMarchThreads;
-            if (x >= Nx) break;
-            const cd a0 = d0[x], ag = dg[x], e = ec[m];
+        for (int m = 0; m < M; ++m) {
+            const cd a0 = lds_b[o0 + xs[m]], ag = lds_b[o0 + P + xs[m]];
+            const cd e = ec[m];
             const cd cb = cmul(A, e);          // ds sbar e^{+iH}
             const cd ch = cconjmul(e, B);      // ds sh  e^{-iH}
             const cd p = cadd(a0, cmul(cb, ag));
             const cd nq = cmul(ch, a0);
             const cd q = {ag.r - nq.r, ag.i - nq.i};
-            n0[x + 1 < Nx ? x + 1 : 0] = cmul(e0, p);
-            ng[x > 0 ? x - 1 : Nx - 1] = cmul(q, ieg);
+            lds_b[n0 + to0[m]] = cmul(e0, p);
+            lds_b[n0 + P + tog[m]] = cmul(q, ieg);
         }
-        __syncthreads();
+        lds_barrier();
         cur = 1 - cur;
     }
-    const cd *dg = Dg[cur];
+    const int og = cur * 2 * P + P;
     for (int x = tid; x < Nx; x += kMarchThreads)
         out[(size_t)s * out_stride + (size_t)j * Nx + x] =
-            cscale(dg[x], out_scale);
+            cscale(lds_b[og + x], out_scale);
+}
+
+using MarchFn = void (*)(BraggK, const double *, const cd *, const cd *,
+                         const cd *, size_t, cd *, size_t, double,
+                         const int *);
+
+template <int... Ms>
+static MarchFn march_fn(int M) {
+    MarchFn f = nullptr;
+    ((M == Ms ? (f = bragg_march_kernel<Ms>, 0) : 0), ...);
+    return f;
 }
 
 static int launch_march(const BraggK &k, const double *s_dev, const cd *E,
                         const cd *inc, const cd *vin, size_t vin_stride,
                         cd *out, size_t out_stride, double out_scale,
                         const int *active, hipStream_t stream) {
-    const size_t lds = (size_t)4 * k.Nx * sizeof(cd);
+    const size_t lds = (size_t)4 * (k.Nx + 1) * sizeof(cd);
+    const int M = (k.Nx + kMarchThreads - 1) / kMarchThreads;
+    MarchFn fn = march_fn<1, 2, 3, 4, 5, 6, 7, 8>(M);
+    if (!fn) return fail(DYF_EINVAL, "Nx=%d exceeds %d", k.Nx,
+                         kMarchThreads * kMaxPerThread);
     static std::once_flag once;
     std::call_once(once, [] {
-        (void)hipFuncSetAttribute((const void *)bragg_march_kernel,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  160 * 1024);
+        for (int m = 1; m <= kMaxPerThread; ++m)
+            (void)hipFuncSetAttribute(
+                (const void *)march_fn<1, 2, 3, 4, 5, 6, 7, 8>(m),
+                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipGetLastError();
     });
-    hipLaunchKernelGGL(bragg_march_kernel, dim3(k.nsys * k.nyc),
-                       dim3(kMarchThreads), lds, stream, k, s_dev, E, inc,
-                       vin, vin_stride, out, out_stride, out_scale, active);
+    hipLaunchKernelGGL(fn, dim3(k.nsys * k.nyc), dim3(kMarchThreads), lds,
+                       stream, k, s_dev, E, inc, vin, vin_stride, out,
+                       out_stride, out_scale, active);
     DYF_LAUNCHED("bragg_march_kernel");
     return DYF_OK;
 }
@@ -399,15 +427,23 @@ __device__ void lartg(cd f, cd g, double *c, cd *s, cd *r) {
 // After CGS2 of column `col`: h[col][0..col] = h1 + h2, h[col][col+1] =
 // ||w||, breakdown test, past and new Givens rotations, S update, presid.
 // inv[s] receives 1/||w|| (0 on breakdown) for the basis store.
-__global__ void arnoldi_finish_kernel(int nsys, int restart, cd *H, cd *G,
-                                      cd *S, const cd *h1st, const cd *h2nd,
-                                      const cd *nrm_w, SysState *st,
-                                      double *inv, const int *active,
-                                      double eps) {
+// Column `col` of every active system: Hessenberg column from the two
+// CGS2 passes, breakdown test against h0 = ||w|| before them (nrm0),
+// the stored Givens rotations and a new one, the residual estimate, and
+// SciPy's inner-loop exit (presid <= ptol, breakdown or the last column),
+// which clears the system's active flag; `still` is set while any system
+// continues, so the host reads one int per column.
+__global__ void arnoldi_finish_kernel(int nsys, int restart, int col, cd *H,
+                                      cd *G, cd *S, const cd *h1st,
+                                      const cd *h2nd, const cd *nrm_w,
+                                      const cd *nrm0, SysState *st,
+                                      double *inv, int *active, double eps,
+                                      int *still) {
     const int s = blockIdx.x * 64 + threadIdx.x;
     if (s >= nsys || !active[s]) return;
     SysState &q = st[s];
-    const int col = q.col;
+    q.col = col;
+    q.h0 = sqrt(nrm0[s].r);
     cd *h = H + (size_t)s * restart * (restart + 1) + (size_t)col * (restart + 1);
     cd *g = G + (size_t)s * restart * 2;
     cd *Sv = S + (size_t)s * (restart + 1);
@@ -450,7 +486,15 @@ __global__ void arnoldi_finish_kernel(int nsys, int restart, cd *H, cd *G,
     q.presid = hypot(tmp.r, tmp.i);
     q.breakdown = breakdown;
     q.iters += 1;
+    if (q.presid <= q.ptol || breakdown || col == restart - 1) {
+        q.inner_done = 1;
+        active[s] = 0;
+    } else {
+        atomicOr(still, 1);
+    }
 }
+
+__global__ void zero_int_kernel(int *p) { *p = 0; }
 
 // Upper-triangular solve of the Hessenberg least squares (SciPy's loop),
 // y stored per system; ncol = col + 1 columns.
@@ -502,10 +546,11 @@ __global__ void exit_wave_kernel(const cd *__restrict__ v,
 
 // ---- workspace layout --------------------------------------------------
 struct Work {
-    cd *V, *b, *r, *w, *av, *H, *G, *S, *Y, *h1, *h2, *nrm, *pinv, *part;
+    cd *V, *b, *r, *w, *av, *H, *G, *S, *Y, *h1, *h2, *nrm, *nrm0, *pinv,
+        *part;
     double *inv;
     SysState *st;
-    int *active;
+    int *active, *still;
     size_t bytes;
 };
 
@@ -539,11 +584,13 @@ static Work carve(void *base, int nsys, size_t N, int restart, int Nx) {
     w.h1 = (cd *)take(sizeof(cd) * nsys * (R + 1));
     w.h2 = (cd *)take(sizeof(cd) * nsys * (R + 1));
     w.nrm = (cd *)take(sizeof(cd) * nsys);
+    w.nrm0 = (cd *)take(sizeof(cd) * nsys);
     w.pinv = (cd *)take(sizeof(cd) * nsys * Nx);
     w.part = (cd *)take(sizeof(cd) * nsys * (R + 2) * P);
     w.inv = (double *)take(sizeof(double) * nsys);
     w.st = (SysState *)take(sizeof(SysState) * nsys);
     w.active = (int *)take(sizeof(int) * nsys);
+    w.still = (int *)take(sizeof(int));
     w.bytes = off;
     return w;
 }
@@ -636,7 +683,7 @@ __global__ __launch_bounds__(kFpThreads) void bragg_fixed_point_kernel(
                 en[m] = Es[(size_t)(z + 1) * zs + x];
                 gn[m] = dg[(size_t)(z + 1) * Nx + x];
             }
-            __syncthreads();
+            lds_barrier();
 #pragma unroll
             for (int m = 0; m < kMaxPerThread; ++m) {
                 const int x = tid + m * kFpThreads;
@@ -669,7 +716,7 @@ __global__ __launch_bounds__(kFpThreads) void bragg_fixed_point_kernel(
                 en[m] = Es[(size_t)z * zs + x];
                 gn[m] = d0[(size_t)z * Nx + x];
             }
-            __syncthreads();
+            lds_barrier();
 #pragma unroll
             for (int m = 0; m < kMaxPerThread; ++m) {
                 const int x = tid + m * kFpThreads;
@@ -972,11 +1019,9 @@ extern "C" int dyf_bragg_solve(const dyf_bragg_args *a, const double *s_dev,
             rc = apply_precon(k, W.pinv, W.av, W.w, stream);
             if (rc) return rc;
             // h0 = ||w||; CGS2 against v_0..v_col
-            rc = dots(W.w, N, 0, 1, W.w, N, N, W.nrm, 1, W.part, nsys,
+            rc = dots(W.w, N, 0, 1, W.w, N, N, W.nrm0, 1, W.part, nsys,
                       W.active, stream);
             if (rc) return rc;
-            DYF_HIP(hipMemcpyAsync(nrm.data(), W.nrm, sizeof(cd) * nsys,
-                                   hipMemcpyDeviceToHost, stream));
             for (int pass = 0; pass < 2; ++pass) {
                 cd *hp = pass == 0 ? W.h1 : W.h2;
                 rc = dots(W.V, Vsys, N, col + 1, W.w, N, N, hp, R + 1, W.part,
@@ -987,41 +1032,28 @@ extern "C" int dyf_bragg_solve(const dyf_bragg_args *a, const double *s_dev,
                     W.active);
                 DYF_LAUNCHED("combine_kernel");
             }
-            DYF_HIP(hipStreamSynchronize(stream));
-            for (int s = 0; s < nsys; ++s)
-                if (act[s]) st[s].h0 = sqrt(nrm[s].r), st[s].col = col;
-            rc = push_state();
-            if (rc) return rc;
             rc = dots(W.w, N, 0, 1, W.w, N, N, W.nrm, 1, W.part, nsys,
                       W.active, stream);
             if (rc) return rc;
+            zero_int_kernel<<<1, 1, 0, stream>>>(W.still);
+            DYF_LAUNCHED("zero_int_kernel");
             arnoldi_finish_kernel<<<(nsys + 63) / 64, 64, 0, stream>>>(
-                nsys, R, W.H, W.G, W.S, W.h1, W.h2, W.nrm, W.st, W.inv,
-                W.active, eps);
+                nsys, R, col, W.H, W.G, W.S, W.h1, W.h2, W.nrm, W.nrm0, W.st,
+                W.inv, W.active, eps, W.still);
             DYF_LAUNCHED("arnoldi_finish_kernel");
-            if (col + 1 <= R) {
-                scale_copy_kernel<<<vgrid, 256, 0, stream>>>(
-                    W.w, N, W.V + (size_t)(col + 1) * N, Vsys, W.inv, N,
-                    W.active);
-                DYF_LAUNCHED("scale_copy_kernel");
-            }
-            rc = pull_state();
-            if (rc) return rc;
+            // v_{col+1} = w / h1 for the systems still iterating
+            scale_copy_kernel<<<vgrid, 256, 0, stream>>>(
+                W.w, N, W.V + (size_t)(col + 1) * N, Vsys, W.inv, N,
+                W.active);
+            DYF_LAUNCHED("scale_copy_kernel");
             int still = 0;
-            for (int s = 0; s < nsys; ++s) {
-                if (!act[s]) continue;
-                if (st[s].presid <= st[s].ptol || st[s].breakdown ||
-                    col == R - 1) {
-                    st[s].inner_done = 1;
-                    act[s] = 0;
-                } else {
-                    still = 1;
-                }
-            }
-            rc = push_state();
-            if (rc) return rc;
+            DYF_HIP(hipMemcpyAsync(&still, W.still, sizeof(int),
+                                   hipMemcpyDeviceToHost, stream));
+            DYF_HIP(hipStreamSynchronize(stream));
             if (!still) break;
         }
+        rc = pull_state();
+        if (rc) return rc;
         // solve the small least-squares problems and update x
         for (int s = 0; s < nsys; ++s) act[s] = !st[s].done;
         rc = push_state();

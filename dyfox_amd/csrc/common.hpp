@@ -64,6 +64,17 @@ struct SegParams {
 int prepare_segments(const double *segs, int nseg, const double *g_vec,
                      double nu, SegParams *out);
 
+// Workgroup barrier for LDS traffic only: waits for this wave's LDS
+// operations, then s_barrier.  __syncthreads() also fences global memory,
+// which drains the vector-memory counter and so waits for every global
+// load still in flight - including loads issued early to prefetch the
+// next step - on every step.  Only for loops whose cross-thread data goes
+// through LDS (global memory read-only, or written and read back by the
+// same thread).
+__device__ __forceinline__ void lds_barrier() {
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 // H = 2 pi g.u at (x, y, z), segments summed in list order (FP64).
 __device__ __forceinline__ double volterra_h(const SegParams *__restrict__ s,
                                              int nseg, double x, double y,
