@@ -332,6 +332,16 @@ __device__ __forceinline__ void fast_row(const FastArgs &a, int r, int ie) {
 // SIMD) this measured faster (cfg4 mixed 4.80 s -> 4.10 s); in the
 // stand-alone kernel the same source measured 5-7 % slower than the
 // rolled fast_row above, so the two are kept apart.
+// The persistent kernel reads its table, coefficient and output pointers
+// from descriptors in device memory, so the compiler cannot tell they are
+// global and would access them as flat memory, whose loads also count
+// against the LDS counter every step's lane exchange waits on.  These
+// casts restate the global address space.
+template <typename T>
+using gptr = const T __attribute__((address_space(1))) *;
+template <typename T>
+using gptr_w = T __attribute__((address_space(1))) *;
+
 template <int C, int PREC>
 __device__ __forceinline__ void fast_row_unrolled(const FastArgs &a, int r,
                                                   int ie) {
@@ -352,21 +362,23 @@ __device__ __forceinline__ void fast_row_unrolled(const FastArgs &a, int r,
         Xi[i] = Yr[i] = Yi[i] = T(0);
         if constexpr (COMP) Xlr[i] = Xli[i] = Ylr[i] = Yli[i] = T(0);
     }
-    const cx<T> *E = (const cx<T> *)a.phase + (size_t)r * a.nsteps * kSlice +
-                     lane;
+    gptr<cx<T>> E = (gptr<cx<T>>)a.phase + (size_t)r * a.nsteps * kSlice + lane;
+    gptr<CoefD> cdg = (gptr<CoefD>)a.cd;
+    gptr<CoefF> cfg = (gptr<CoefF>)a.cf;
 
     auto step = [&](int k) {
         cx<T> Ec[C];
-        const cx<T> *Ek = E + (size_t)k * kSlice;
+        gptr<cx<T>> Ek = E + (size_t)k * kSlice;
 #pragma unroll
-        for (int i = 0; i < C; ++i) Ec[i] = Ek[i * kWave];
+        for (int i = 0; i < C; ++i)
+            Ec[i] = cx<T>{Ek[i * kWave].r, Ek[i * kWave].i};
         T cr, ci, br, bi;
         if constexpr (sizeof(T) == 8) {
-            const CoefD c = a.cd[(size_t)ie * a.nsteps + k];
-            cr = c.cr, ci = c.ci, br = c.br, bi = c.bi;
+            gptr<CoefD> c = cdg + (size_t)ie * a.nsteps + k;
+            cr = c->cr, ci = c->ci, br = c->br, bi = c->bi;
         } else {
-            const CoefF c = a.cf[(size_t)ie * a.nsteps + k];
-            cr = c.cr, ci = c.ci, br = c.br, bi = c.bi;
+            gptr<CoefF> c = cfg + (size_t)ie * a.nsteps + k;
+            cr = c->cr, ci = c->ci, br = c->br, bi = c->bi;
         }
         // inflow across lane boundaries (old values)
         const T xin_r = from_right(Xr[0], lane), xin_i = from_right(Xi[0], lane);
@@ -430,7 +442,8 @@ __device__ __forceinline__ void fast_row_unrolled(const FastArgs &a, int r,
     }
     for (; k < a.nsteps; ++k) step(k);
 
-    const double qr = a.qN[2 * ie], qi = a.qN[2 * ie + 1];
+    gptr<double> qN = (gptr<double>)a.qN;
+    const double qr = qN[2 * ie], qi = qN[2 * ie + 1];
     const size_t row = ((size_t)ie * a.Ny + iy) * a.Nx;
 #pragma unroll
     for (int i = 0; i < C; ++i) {
@@ -439,10 +452,13 @@ __device__ __forceinline__ void fast_row_unrolled(const FastArgs &a, int r,
         double yr = (double)Yr[i], yi = (double)Yi[i];
         if constexpr (COMP) yr += (double)Ylr[i], yi += (double)Yli[i];
         const double o_r = qr * yr - qi * yi, o_i = qr * yi + qi * yr;
-        if constexpr (PREC == DYF_FP64)
-            ((cx<double> *)a.out)[row + x] = cx<double>{o_r, o_i};
-        else
-            ((cx<float> *)a.out)[row + x] = cx<float>{(float)o_r, (float)o_i};
+        if constexpr (PREC == DYF_FP64) {
+            gptr_w<cx<double>> o = (gptr_w<cx<double>>)a.out + row + x;
+            o->r = o_r, o->i = o_i;
+        } else {
+            gptr_w<cx<float>> o = (gptr_w<cx<float>>)a.out + row + x;
+            o->r = (float)o_r, o->i = (float)o_i;
+        }
     }
 }
 
