@@ -57,6 +57,8 @@ struct SegParams {
     double edge;            // 1 when b_perp > 1e-20
     double wind;            // g.b rounded, when it is an integer in
                             // [-8, 8] (within 1e-9); NaN otherwise
+    double f1x, f1y, f1z;   // e1 - (e1.xi) xi: d1 = f1.(r - c) equals
+    double f2x, f2y, f2z;   // e1.(r - c projected off xi); same for e2
 };
 
 // Reduces raw (ra, rb, b) segments to SegParams; segments shorter than
@@ -100,37 +102,94 @@ __device__ __forceinline__ double volterra_h(const SegParams *__restrict__ s,
     return H;
 }
 
+// Natural logarithm of a positive, normal x, to about one ulp:
+// x = 2^e m with m in [sqrt(1/2), sqrt(2)), and log m = 2 atanh(s),
+// s = (m - 1)/(m + 1), |s| <= 0.1716, summed as the odd series
+// 2 (s + s^3/3 + ... + s^19/19); the first dropped term is below 2^-55
+// of the sum.  m - 1 is exact (Sterbenz), 1/(m + 1) takes one Newton step
+// from v_rcp_f64 and s one residual correction.
+__device__ __forceinline__ double log_pos(double x) {
+    int e;
+    double m = frexp(x, &e);                  // m in [1/2, 1)
+    const bool lo = m < 0.70710678118654752440;
+    m = lo ? m + m : m;
+    e = lo ? e - 1 : e;
+    const double f = m - 1.0, d = m + 1.0;
+    double r = __builtin_amdgcn_rcp(d);
+    r = fma(r, fma(-d, r, 1.0), r);
+    double sv = f * r;
+    sv = fma(r, fma(-sv, d, f), sv);
+    const double u = sv * sv;
+    double q = 1.0 / 19.0;
+    q = fma(q, u, 1.0 / 17.0);
+    q = fma(q, u, 1.0 / 15.0);
+    q = fma(q, u, 1.0 / 13.0);
+    q = fma(q, u, 1.0 / 11.0);
+    q = fma(q, u, 1.0 / 9.0);
+    q = fma(q, u, 1.0 / 7.0);
+    q = fma(q, u, 1.0 / 5.0);
+    q = fma(q, u, 1.0 / 3.0);
+    const double two_s = sv + sv;
+    const double logm = fma(two_s * u, q, two_s);
+    // ln 2 = kLn2Hi + kLn2Lo to 3e-31: kLn2Hi is ln 2 rounded to 42
+    // bits, so e * kLn2Hi is exact for every double exponent e.
+    constexpr double kLn2Hi = 0x1.62e42fefa3800p-1;
+    constexpr double kLn2Lo = 0x1.ef35793c76730p-45;
+    const double de = (double)e;
+    return fma(de, kLn2Hi, fma(de, kLn2Lo, logm));
+}
+
 // e^{iH} at (x, y, z) as P e^{iR}: P = prod (w/|w|)^(g.b) over the
 // segments whose g.b is an integer (w = d1 + i d2, so (w/|w|)^n =
 // e^{i n atan2(d2, d1)} without the atan2), R = the remaining terms (the
 // atan2 winding of non-integer g.b and the edge terms).  Equal to
 // e^{i volterra_h} up to rounding; P is a unit complex number.
+// One reciprocal of r^2 = d1^2 + d2^2 serves both: with
+// (c, s) = ((d1^2 - d2^2), 2 d1 d2) / r^2 = (w/|w|)^2, the edge terms are
+// k_xy s/2 + k_dd c, scaled by r^2/a^2 inside the core floor a, and even
+// windings are powers of (c, s) (odd ones take one more factor w/|w|).
+// A voxel on the line (r^2 = 0) gets the factor 1 (atan2(0, 0) = 0) and
+// no k_xy/k_dd term.  The core floor is taken as at least the smallest
+// normal double, so a zero core radius leaves a voxel on the line finite
+// (log of 2^-1022) where the unfloored formula has 0/0.
 __device__ __forceinline__ void volterra_phase(const SegParams *__restrict__ s,
                                                int nseg, double x, double y,
                                                double z, double a2,
                                                double &Pr, double &Pi,
                                                double &R) {
     Pr = 1.0, Pi = 0.0, R = 0.0;
+    a2 = fmax(a2, 0x1p-1022);
+    const double inv_a2 = 1.0 / a2;
     for (int n = 0; n < nseg; ++n) {
         const SegParams &p = s[n];
         const double X = x - p.cx, Y = y - p.cy, Z = z - p.cz;
-        const double eta = X * p.xx + Y * p.xy + Z * p.xz;
-        const double Px = X - eta * p.xx, Py = Y - eta * p.xy,
-                     Pz = Z - eta * p.xz;
-        const double d1 = Px * p.e1x + Py * p.e1y + Pz * p.e1z;
-        const double d2 = Px * p.e2x + Py * p.e2y + Pz * p.e2z;
-        const double r2 = d1 * d1 + d2 * d2;
+        const double d1 = X * p.f1x + Y * p.f1y + Z * p.f1z;
+        const double d2 = X * p.f2x + Y * p.f2y + Z * p.f2z;
+        const double r2 = fma(d1, d1, d2 * d2);
+        const bool on = r2 >= 0x1p-1022;
+        double q = __builtin_amdgcn_rcp(r2);
+        q = fma(q, fma(-r2, q, 1.0), q);
+        q = fma(q, fma(-r2, q, 1.0), q);
+        const double c2 = on ? fma(d1, d1, -d2 * d2) * q : 1.0;
+        const double sh = on ? d1 * d2 * q : 0.0;        // s / 2
         if (p.wind == p.wind) {
             int m = (int)p.wind;
-            if (m != 0 && r2 > 0.0) {      // atan2(0, 0) = 0: factor 1
-                const double rs = rsqrt(r2);
-                const double ur = d1 * rs;
-                const double ui = m < 0 ? -d2 * rs : d2 * rs;
+            if (m != 0) {
+                const double sg = m < 0 ? -1.0 : 1.0;
                 m = m < 0 ? -m : m;
-                double wr = ur, wi = ui;
-                for (int j = 1; j < m; ++j) {
-                    const double tr = wr * ur - wi * ui;
-                    wi = wr * ui + wi * ur;
+                const double br = c2, bi = sg * (sh + sh);
+                double wr, wi;
+                if (m & 1) {
+                    const double rs = rsqrt(r2);
+                    wr = on ? d1 * rs : 1.0;
+                    wi = on ? sg * d2 * rs : 0.0;
+                } else {
+                    wr = br, wi = bi;
+                    m -= 2;
+                }
+                for (; m >= 2; m -= 2) {
+                    const double tr = wr * br - wi * bi;
+                    wi = wr * bi + wi * br;
                     wr = tr;
                 }
                 const double tr = Pr * wr - Pi * wi;
@@ -141,10 +200,9 @@ __device__ __forceinline__ void volterra_phase(const SegParams *__restrict__ s,
             R += p.c_atan * atan2(d2, d1);
         }
         if (p.edge != 0.0) {
-            const double r2c = fmax(r2, a2);
-            const double inv = 1.0 / r2c;
-            R += (p.k_xy * (d1 * d2) + p.k_dd * (d1 * d1 - d2 * d2)) * inv +
-                 p.k_log * log(r2c);
+            const double t = fma(p.k_xy, sh, p.k_dd * c2);
+            const double core = fmin(r2 * inv_a2, 1.0);
+            R += fma(t, core, p.k_log * log_pos(fmax(r2, a2)));
         }
     }
 }

@@ -81,6 +81,11 @@ int prepare_segments(const double *segs, int nseg, const double *g,
         p.k_dd = -kTwoPi * ge2 * c1 / (4.0 * onu);
         p.k_log = -kTwoPi * ge2 * c1 * (1.0 - 2.0 * nu) / (4.0 * onu);
         p.edge = b_perp > 1e-20 ? 1.0 : 0.0;
+        const double x1 = dot3(e1, xi), x2 = dot3(e2, xi);
+        p.f1x = e1[0] - x1 * xi[0], p.f1y = e1[1] - x1 * xi[1],
+        p.f1z = e1[2] - x1 * xi[2];
+        p.f2x = e2[0] - x2 * xi[0], p.f2y = e2[1] - x2 * xi[1],
+        p.f2z = e2[2] - x2 * xi[2];
         const double nw = std::nearbyint(p.c_atan);
         p.wind = std::fabs(p.c_atan - nw) <=
                              1e-9 * std::fmax(1.0, std::fabs(p.c_atan)) &&
