@@ -55,8 +55,10 @@ struct SegParams {
     double k_dd;            // -2 pi g.e2 c1 / (4(1-nu))       * (d1^2-d2^2)/r2c
     double k_log;           // -2 pi g.e2 c1 (1-2nu)/(4(1-nu)) * ln r2c
     double edge;            // 1 when b_perp > 1e-20
-    double wind;            // g.b rounded, when it is an integer in
-                            // [-8, 8] (within 1e-9); NaN otherwise
+    int32_t nwind;          // g.b rounded, when it is an integer in
+                            // [-8, 8] (within 1e-9); 0 otherwise
+    int32_t frac;           // 1 when g.b is not such an integer (its
+                            // winding is taken with atan2)
     double f1x, f1y, f1z;   // e1 - (e1.xi) xi: d1 = f1.(r - c) equals
     double f2x, f2y, f2z;   // e1.(r - c projected off xi); same for e2
 };
@@ -102,41 +104,58 @@ __device__ __forceinline__ double volterra_h(const SegParams *__restrict__ s,
     return H;
 }
 
+// d = a b + c in one v_fma_f64 with c in a scalar register.  Written
+// out for polynomial steps: the compiler otherwise selects the two-address
+// v_fmac_f64 and copies the (loop-invariant) coefficient into the
+// destination before every use.
+__device__ __forceinline__ double fma_sc(double a, double b, double c) {
+    double d;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "s"(c));
+    return d;
+}
+
+// Table for log_tab: entry j covers mantissas m in [1 + j/128,
+// 1 + (j+1)/128) with c = 1/(1 + (j + 1/2)/128) (rounded) and
+// nl = -log(c) of that rounded c.  Built per workgroup in LDS by
+// fill_log_table (one library log per entry).
+constexpr int kLogTab = 128;
+struct alignas(16) LogEntry {
+    double c, nl;
+};
+
+__device__ __forceinline__ void fill_log_table(LogEntry *tab) {
+    for (int j = threadIdx.x; j < kLogTab; j += blockDim.x) {
+        const double c = 1.0 / (1.0 + ((double)j + 0.5) / kLogTab);
+        tab[j] = LogEntry{c, -log(c)};
+    }
+}
+
 // Natural logarithm of a positive, normal x, to about one ulp:
-// x = 2^e m with m in [sqrt(1/2), sqrt(2)), and log m = 2 atanh(s),
-// s = (m - 1)/(m + 1), |s| <= 0.1716, summed as the odd series
-// 2 (s + s^3/3 + ... + s^19/19); the first dropped term is below 2^-55
-// of the sum.  m - 1 is exact (Sterbenz), 1/(m + 1) takes one Newton step
-// from v_rcp_f64 and s one residual correction.
-__device__ __forceinline__ double log_pos(double x) {
-    int e;
-    double m = frexp(x, &e);                  // m in [1/2, 1)
-    const bool lo = m < 0.70710678118654752440;
-    m = lo ? m + m : m;
-    e = lo ? e - 1 : e;
-    const double f = m - 1.0, d = m + 1.0;
-    double r = __builtin_amdgcn_rcp(d);
-    r = fma(r, fma(-d, r, 1.0), r);
-    double sv = f * r;
-    sv = fma(r, fma(-sv, d, f), sv);
-    const double u = sv * sv;
-    double q = 1.0 / 19.0;
-    q = fma(q, u, 1.0 / 17.0);
-    q = fma(q, u, 1.0 / 15.0);
-    q = fma(q, u, 1.0 / 13.0);
-    q = fma(q, u, 1.0 / 11.0);
-    q = fma(q, u, 1.0 / 9.0);
-    q = fma(q, u, 1.0 / 7.0);
-    q = fma(q, u, 1.0 / 5.0);
-    q = fma(q, u, 1.0 / 3.0);
-    const double two_s = sv + sv;
-    const double logm = fma(two_s * u, q, two_s);
+// x = 2^e m, m in [1, 2); with the table entry (c, nl) of m's top seven
+// fraction bits, r = m c - 1 (one fma; |r| <= 2^-8) and
+// log x = e ln 2 + nl + log1p(r), log1p(r) = r - r^2/2 + ... - r^6/6 +
+// r^7/7 (the first dropped term is below 2e-18 of r).
+__device__ __forceinline__ double log_tab(double x,
+                                          const LogEntry *__restrict__ tab) {
+    const unsigned long long bits = __double_as_longlong(x);
+    const unsigned hi = (unsigned)(bits >> 32);
+    const int e = (int)(hi >> 20) - 1023;
+    const LogEntry t = tab[(hi >> 13) & (kLogTab - 1)];
+    const double m = __longlong_as_double(
+        (bits & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull);
+    const double r = fma(m, t.c, -1.0);
+    double q = fma(r, 1.0 / 7.0, -1.0 / 6.0);
+    q = fma_sc(q, r, 1.0 / 5.0);
+    q = fma_sc(q, r, -1.0 / 4.0);
+    q = fma_sc(q, r, 1.0 / 3.0);
+    q = fma_sc(q, r, -1.0 / 2.0);
+    const double lm = fma(r * r, q, r) + t.nl;
     // ln 2 = kLn2Hi + kLn2Lo to 3e-31: kLn2Hi is ln 2 rounded to 42
     // bits, so e * kLn2Hi is exact for every double exponent e.
     constexpr double kLn2Hi = 0x1.62e42fefa3800p-1;
     constexpr double kLn2Lo = 0x1.ef35793c76730p-45;
     const double de = (double)e;
-    return fma(de, kLn2Hi, fma(de, kLn2Lo, logm));
+    return fma(de, kLn2Hi, fma(de, kLn2Lo, lm));
 }
 
 // e^{iH} at (x, y, z) as P e^{iR}: P = prod (w/|w|)^(g.b) over the
@@ -152,59 +171,146 @@ __device__ __forceinline__ double log_pos(double x) {
 // no k_xy/k_dd term.  The core floor is taken as at least the smallest
 // normal double, so a zero core radius leaves a voxel on the line finite
 // (log of 2^-1022) where the unfloored formula has 0/0.
-__device__ __forceinline__ void volterra_phase(const SegParams *__restrict__ s,
-                                               int nseg, double x, double y,
-                                               double z, double a2,
-                                               double &Pr, double &Pi,
-                                               double &R) {
-    Pr = 1.0, Pi = 0.0, R = 0.0;
-    a2 = fmax(a2, 0x1p-1022);
+//
+// NP points share (x, y) and differ in z: a segment's parameters, its
+// (uniform) branches and the x, y parts of d1, d2 are taken once for all
+// NP.  `tab` is the LDS table of log_tab.
+template <int NP>
+__device__ __forceinline__ void volterra_phase_z(
+    const SegParams *__restrict__ s, int nseg, double x, double y,
+    const double (&z)[NP], double a2, const LogEntry *__restrict__ tab,
+    double (&Pr)[NP], double (&Pi)[NP], double (&R)[NP]) {
+    constexpr double kTiny = 0x1p-1022;
+#pragma unroll
+    for (int j = 0; j < NP; ++j) Pr[j] = 1.0, Pi[j] = 0.0, R[j] = 0.0;
+    a2 = fmax(a2, kTiny);
     const double inv_a2 = 1.0 / a2;
     for (int n = 0; n < nseg; ++n) {
         const SegParams &p = s[n];
-        const double X = x - p.cx, Y = y - p.cy, Z = z - p.cz;
-        const double d1 = X * p.f1x + Y * p.f1y + Z * p.f1z;
-        const double d2 = X * p.f2x + Y * p.f2y + Z * p.f2z;
-        const double r2 = fma(d1, d1, d2 * d2);
-        const bool on = r2 >= 0x1p-1022;
-        double q = __builtin_amdgcn_rcp(r2);
-        q = fma(q, fma(-r2, q, 1.0), q);
-        q = fma(q, fma(-r2, q, 1.0), q);
-        const double c2 = on ? fma(d1, d1, -d2 * d2) * q : 1.0;
-        const double sh = on ? d1 * d2 * q : 0.0;        // s / 2
-        if (p.wind == p.wind) {
-            int m = (int)p.wind;
-            if (m != 0) {
-                const double sg = m < 0 ? -1.0 : 1.0;
-                m = m < 0 ? -m : m;
-                const double br = c2, bi = sg * (sh + sh);
+        const double X = x - p.cx, Y = y - p.cy;
+        const double b1 = fma(Y, p.f1y, X * p.f1x);
+        const double b2 = fma(Y, p.f2y, X * p.f2x);
+        double d1[NP], d2[NP], r2[NP], c2[NP], sh[NP];
+#pragma unroll
+        for (int j = 0; j < NP; ++j) {
+            const double Z = z[j] - p.cz;
+            d1[j] = fma(Z, p.f1z, b1);
+            d2[j] = fma(Z, p.f2z, b2);
+            r2[j] = fma(d1[j], d1[j], d2[j] * d2[j]);
+            const double rc = fmax(r2[j], kTiny);
+            double q = __builtin_amdgcn_rcp(rc);
+            q = fma(q, fma(-rc, q, 1.0), q);
+            q = fma(q, fma(-rc, q, 1.0), q);
+            c2[j] = fma(d1[j], d1[j], -d2[j] * d2[j]) * q;    // 0 on the line
+            sh[j] = d1[j] * d2[j] * q;                       // s / 2
+        }
+        const int mw = p.nwind;
+        if (mw != 0) {
+            const double sg = mw < 0 ? -1.0 : 1.0;
+            const int m = mw < 0 ? -mw : mw;
+#pragma unroll
+            for (int j = 0; j < NP; ++j) {
+                const bool on = r2[j] >= kTiny;
+                const double br = on ? c2[j] : 1.0, bi = sg * (sh[j] + sh[j]);
                 double wr, wi;
+                int k;
                 if (m & 1) {
-                    const double rs = rsqrt(r2);
-                    wr = on ? d1 * rs : 1.0;
-                    wi = on ? sg * d2 * rs : 0.0;
+                    const double rs = rsqrt(r2[j]);
+                    wr = on ? d1[j] * rs : 1.0;
+                    wi = on ? sg * d2[j] * rs : 0.0;
+                    k = m - 1;
                 } else {
                     wr = br, wi = bi;
-                    m -= 2;
+                    k = m - 2;
                 }
-                for (; m >= 2; m -= 2) {
+                for (; k >= 2; k -= 2) {
                     const double tr = wr * br - wi * bi;
                     wi = wr * bi + wi * br;
                     wr = tr;
                 }
-                const double tr = Pr * wr - Pi * wi;
-                Pi = Pr * wi + Pi * wr;
-                Pr = tr;
+                const double tr = Pr[j] * wr - Pi[j] * wi;
+                Pi[j] = Pr[j] * wi + Pi[j] * wr;
+                Pr[j] = tr;
             }
-        } else {
-            R += p.c_atan * atan2(d2, d1);
+        } else if (p.frac) {
+#pragma unroll
+            for (int j = 0; j < NP; ++j) R[j] += p.c_atan * atan2(d2[j], d1[j]);
         }
         if (p.edge != 0.0) {
-            const double t = fma(p.k_xy, sh, p.k_dd * c2);
-            const double core = fmin(r2 * inv_a2, 1.0);
-            R += fma(t, core, p.k_log * log_pos(fmax(r2, a2)));
+#pragma unroll
+            for (int j = 0; j < NP; ++j) {
+                const double t = fma(p.k_xy, sh[j], p.k_dd * c2[j]);
+                const double core = fmin(r2[j] * inv_a2, 1.0);
+                R[j] = fma(p.k_log, log_tab(fmax(r2[j], a2), tab), R[j]);
+                R[j] = fma(t, core, R[j]);
+            }
         }
     }
+}
+
+__device__ __forceinline__ void volterra_phase(const SegParams *__restrict__ s,
+                                               int nseg, double x, double y,
+                                               double z, double a2,
+                                               const LogEntry *__restrict__ tab,
+                                               double &Pr, double &Pi,
+                                               double &R) {
+    const double zz[1] = {z};
+    double pr[1], pi[1], rr[1];
+    volterra_phase_z<1>(s, nseg, x, y, zz, a2, tab, pr, pi, rr);
+    Pr = pr[0], Pi = pi[0], R = rr[0];
+}
+
+// The library's sincos out of line, for the rare argument the reduction
+// below does not take: inlined, the compiler runs its Payne-Hanek branch
+// unconditionally on every lane.
+__device__ __attribute__((noinline)) inline cx<double> sincos_lib(double x) {
+    double s, c;
+    sincos(x, &s, &c);
+    return {c, s};
+}
+
+// sin and cos of x to about one ulp: x = n pi/2 + r with n the nearest
+// integer to 2x/pi and pi/2 = kPio2Hi + kPio2Lo (a double-double, to
+// 1e-33 relative; with fma the two subtractions leave r good to an ulp
+// for any |x| < 2^30), then Taylor series on |r| <= pi/4 (sin through
+// r^17/17!, cos through r^16/16!: the first dropped terms are below
+// 1e-19) and the quadrant n mod 4.  Larger |x|, infinities and NaN take
+// the library routine.
+__device__ __forceinline__ void sincos_red(double x, double &sn, double &cs) {
+    if (!(fabs(x) < 0x1p30)) {
+        const cx<double> e = sincos_lib(x);
+        sn = e.i, cs = e.r;
+        return;
+    }
+    constexpr double k2OverPi = 0x1.45f306dc9c883p-1;
+    constexpr double kPio2Hi = 0x1.921fb54442d18p+0;
+    constexpr double kPio2Lo = 0x1.1a62633145c07p-54;
+    const double n = rint(x * k2OverPi);
+    double r = fma(-n, kPio2Hi, x);
+    r = fma(-n, kPio2Lo, r);
+    const double u = r * r;
+    double ps = fma(u, 1.0 / 355687428096000.0,                // 1/17!
+                    -1.0 / 1307674368000.0);                    // 1/15!
+    ps = fma_sc(ps, u, 1.0 / 6227020800.0);                     // 1/13!
+    ps = fma_sc(ps, u, -1.0 / 39916800.0);                      // 1/11!
+    ps = fma_sc(ps, u, 1.0 / 362880.0);                         // 1/9!
+    ps = fma_sc(ps, u, -1.0 / 5040.0);                          // 1/7!
+    ps = fma_sc(ps, u, 1.0 / 120.0);                            // 1/5!
+    ps = fma_sc(ps, u, -1.0 / 6.0);                             // 1/3!
+    const double sr = fma(r * u, ps, r);
+    double pc = fma(u, 1.0 / 20922789888000.0,                 // 1/16!
+                    -1.0 / 87178291200.0);                      // 1/14!
+    pc = fma_sc(pc, u, 1.0 / 479001600.0);                      // 1/12!
+    pc = fma_sc(pc, u, -1.0 / 3628800.0);                       // 1/10!
+    pc = fma_sc(pc, u, 1.0 / 40320.0);                          // 1/8!
+    pc = fma_sc(pc, u, -1.0 / 720.0);                           // 1/6!
+    pc = fma_sc(pc, u, 1.0 / 24.0);                             // 1/4!
+    const double cr = fma(u * u, pc, fma(-0.5, u, 1.0));
+    const int qd = (int)(long long)n;
+    const bool swap = qd & 1;
+    const double a = swap ? cr : sr, b = swap ? sr : cr;
+    sn = (qd & 2) ? -a : a;
+    cs = ((qd + 1) & 2) ? -b : b;
 }
 
 // FP32 segment accumulation (the reference's FP32 H grid: each segment's
@@ -237,7 +343,7 @@ __device__ __forceinline__ cx<double> phase_f64(double H) {
 __device__ __forceinline__ cx<double> phase_parts_f64(double Pr, double Pi,
                                                       double R) {
     double s, c;
-    sincos(R, &s, &c);
+    sincos_red(R, s, c);
     return {Pr * c - Pi * s, Pr * s + Pi * c};
 }
 

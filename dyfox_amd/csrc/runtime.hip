@@ -87,11 +87,12 @@ int prepare_segments(const double *segs, int nseg, const double *g,
         p.f2x = e2[0] - x2 * xi[0], p.f2y = e2[1] - x2 * xi[1],
         p.f2z = e2[2] - x2 * xi[2];
         const double nw = std::nearbyint(p.c_atan);
-        p.wind = std::fabs(p.c_atan - nw) <=
-                             1e-9 * std::fmax(1.0, std::fabs(p.c_atan)) &&
-                         std::fabs(nw) <= 8.0
-                     ? nw
-                     : std::nan("");
+        const bool integral =
+            std::fabs(p.c_atan - nw) <=
+                1e-9 * std::fmax(1.0, std::fabs(p.c_atan)) &&
+            std::fabs(nw) <= 8.0;
+        p.nwind = integral ? (int32_t)nw : 0;
+        p.frac = integral ? 0 : 1;
     }
     return kept;
 }

@@ -58,30 +58,57 @@ __global__ __launch_bounds__(256) void volterra_grid_kernel(
 }
 
 // ---- fused e^{iH} table for the fast march ---------------------------
-// Thread t of a (row, step) covers x = lane*C + i with i = t / 64,
-// lane = t % 64, and writes element t of that (row, step) block.
+// Thread t of a (row, step block) covers x = lane*C + i with i = t / 64,
+// lane = t % 64, for kPhaseZ consecutive steps, and writes element t of
+// each of those (row, step) blocks.  Columns past Nx (the lane padding)
+// are written as zeros.
+constexpr int kPhaseZ = 4;
+
 template <typename T>
-__global__ __launch_bounds__(256) void phase_from_segments_kernel(
-    cx<T> *__restrict__ E, int nseg, int C, int Nx, int Ny, int nsteps,
-    int iy_lo, double dx, double dz, double a2) {
+__device__ __forceinline__ void phase_table_points(
+    cx<T> *__restrict__ E, const SegParams *__restrict__ segs, int nseg,
+    int C, int Nx, int Ny, int nsteps, int iy_lo, double dx, double dz,
+    double a2) {
+    __shared__ LogEntry tab[kLogTab];
+    fill_log_table(tab);
+    __syncthreads();
     const int t = blockIdx.x * 256 + threadIdx.x;
-    const int k = blockIdx.y;
+    const int k0 = blockIdx.y * kPhaseZ;
     const int r = blockIdx.z;
     if (t >= C * kWave) return;
     const int i = t / kWave, lane = t % kWave;
     const int x = lane * C + i;
-    cx<T> e{T(0), T(0)};
-    if (x < Nx) {
-        const double xp = ((double)x - (double)Nx / 2.0) * dx;
-        const double yp = (double)(iy_lo + r - Ny / 2) * dx;
-        double Pr, Pi, R;
-        volterra_phase(c_segs, nseg, xp, yp, (double)k * dz, a2, Pr, Pi, R);
-        if constexpr (sizeof(T) == 8)
-            e = phase_parts_f64(Pr, Pi, R);
-        else
-            e = phase_parts_f32(Pr, Pi, R);
+    const size_t stride = (size_t)C * kWave;
+    cx<T> *out = E + ((size_t)r * nsteps + k0) * stride + t;
+    const int nk = min(kPhaseZ, nsteps - k0);
+    if (x >= Nx) {
+#pragma unroll
+        for (int j = 0; j < kPhaseZ; ++j)
+            if (j < nk) out[j * stride] = cx<T>{T(0), T(0)};
+        return;
     }
-    E[((size_t)r * nsteps + k) * (size_t)(C * kWave) + t] = e;
+    const double xp = ((double)x - (double)Nx / 2.0) * dx;
+    const double yp = (double)(iy_lo + r - Ny / 2) * dx;
+    double z[kPhaseZ], Pr[kPhaseZ], Pi[kPhaseZ], R[kPhaseZ];
+#pragma unroll
+    for (int j = 0; j < kPhaseZ; ++j) z[j] = (double)(k0 + j) * dz;
+    volterra_phase_z<kPhaseZ>(segs, nseg, xp, yp, z, a2, tab, Pr, Pi, R);
+#pragma unroll
+    for (int j = 0; j < kPhaseZ; ++j) {
+        if (j >= nk) break;
+        if constexpr (sizeof(T) == 8)
+            out[j * stride] = phase_parts_f64(Pr[j], Pi[j], R[j]);
+        else
+            out[j * stride] = phase_parts_f32(Pr[j], Pi[j], R[j]);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void phase_from_segments_kernel(
+    cx<T> *__restrict__ E, int nseg, int C, int Nx, int Ny, int nsteps,
+    int iy_lo, double dx, double dz, double a2) {
+    phase_table_points<T>(E, c_segs, nseg, C, Nx, Ny, nsteps, iy_lo, dx, dz,
+                          a2);
 }
 
 // Same table with the segments read from device memory (uniform loads),
@@ -92,31 +119,16 @@ __global__ __launch_bounds__(256) void phase_segments_dev_kernel(
     cx<T> *__restrict__ E, const SegParams *__restrict__ segs, int nseg,
     int C, int Nx, int Ny, int nsteps, int iy_lo, double dx, double dz,
     double a2) {
-    const int t = blockIdx.x * 256 + threadIdx.x;
-    const int k = blockIdx.y;
-    const int r = blockIdx.z;
-    if (t >= C * kWave) return;
-    const int i = t / kWave, lane = t % kWave;
-    const int x = lane * C + i;
-    cx<T> e{T(0), T(0)};
-    if (x < Nx) {
-        const double xp = ((double)x - (double)Nx / 2.0) * dx;
-        const double yp = (double)(iy_lo + r - Ny / 2) * dx;
-        double Pr, Pi, R;
-        volterra_phase(segs, nseg, xp, yp, (double)k * dz, a2, Pr, Pi, R);
-        if constexpr (sizeof(T) == 8)
-            e = phase_parts_f64(Pr, Pi, R);
-        else
-            e = phase_parts_f32(Pr, Pi, R);
-    }
-    E[((size_t)r * nsteps + k) * (size_t)(C * kWave) + t] = e;
+    phase_table_points<T>(E, segs, nseg, C, Nx, Ny, nsteps, iy_lo, dx, dz,
+                          a2);
 }
 
 int launch_phase_segments_dev(const dyf_laue_args *a, const PhaseLayout &L,
                               int prec, const SegParams *segs, int nseg,
                               double a2, void *phase, hipStream_t stream) {
     if (L.nrows == 0) return DYF_OK;
-    dim3 grid((L.C * kWave + 255) / 256, L.nsteps, L.nrows);
+    dim3 grid((L.C * kWave + 255) / 256,
+              (L.nsteps + kPhaseZ - 1) / kPhaseZ, L.nrows);
     if (prec == DYF_FP64)
         phase_segments_dev_kernel<double><<<grid, 256, 0, stream>>>(
             (cx<double> *)phase, segs, nseg, L.C, a->Nx, a->Ny, L.nsteps,
@@ -184,12 +196,16 @@ __global__ __launch_bounds__(256) void phase_from_h_kernel(
 __global__ __launch_bounds__(256) void bragg_phase_kernel(
     cx<double> *__restrict__ E, int nseg, int Nx, int ny, double dx,
     double dz, double a2) {
+    __shared__ LogEntry tab[kLogTab];
+    fill_log_table(tab);
+    __syncthreads();
     const int ix = blockIdx.x * 256 + threadIdx.x;
     const int k = blockIdx.y, j = blockIdx.z;
     if (ix >= Nx) return;
     const double x = ((double)ix - (double)Nx / 2.0) * dx;
     double Pr, Pi, R;
-    volterra_phase(c_segs, nseg, x, c_yvals[j], (double)k * dz, a2, Pr, Pi, R);
+    volterra_phase(c_segs, nseg, x, c_yvals[j], (double)k * dz, a2, tab, Pr,
+                   Pi, R);
     E[((size_t)k * ny + j) * Nx + ix] = phase_parts_f64(Pr, Pi, R);
 }
 
@@ -272,7 +288,8 @@ extern "C" int dyf_laue_phase_from_segments(const dyf_laue_args *a, int prec,
     int kept = 0;
     rc = upload_segments(segs, nseg, g, nu, &kept, stream);
     if (rc) return rc;
-    dim3 grid((L.C * kWave + 255) / 256, L.nsteps, L.nrows);
+    dim3 grid((L.C * kWave + 255) / 256,
+              (L.nsteps + kPhaseZ - 1) / kPhaseZ, L.nrows);
     const double a2 = a_core * a_core;
     if (prec == DYF_FP64)
         phase_from_segments_kernel<double><<<grid, 256, 0, stream>>>(
