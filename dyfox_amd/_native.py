@@ -272,9 +272,11 @@ def device_count():
 
 
 def mem_info():
+    """(free, total) device bytes; blocks held by the pool count as free
+    (an allocation that does not fit releases them)."""
     f, t = C.c_size_t(0), C.c_size_t(0)
     call("dyf_mem_info", C.byref(f), C.byref(t))
-    return f.value, t.value
+    return f.value + _pool_bytes, t.value
 
 
 def synchronize():
@@ -310,20 +312,67 @@ def dptr(a):
     return a.ctypes.data_as(dp)
 
 
+# Blocks whose last view was dropped, kept for reuse by size class (the
+# role CuPy's memory pool plays for the reference).  hipFree waits for the
+# whole device, so freeing a solve's output while the next solve is being
+# enqueued would drain the queue every call.  Reuse is safe in stream
+# order: the library's Python calls enqueue on the null stream, and the
+# streams it creates are blocking ones (ordered against the null stream);
+# work on such a stream is synchronized before its buffers are dropped.
+_pool = {}
+_pool_bytes = 0
+
+
+def _size_class(nbytes):
+    """Round up to 256 B below 1 MiB, to a multiple of 1 MiB above."""
+    n = max(int(nbytes), 1)
+    if n < (1 << 20):
+        return (n + 255) & ~255
+    return (n + (1 << 20) - 1) & ~((1 << 20) - 1)
+
+
+def release_pool():
+    """Return every pooled block to the device (hipFree)."""
+    global _pool_bytes
+    if _lib is None:
+        return
+    for size, ptrs in _pool.items():
+        for p in ptrs:
+            _lib.dyf_free(vp(p))
+    _pool.clear()
+    _pool_bytes = 0
+
+
 class _Block:
-    """One device allocation; freed when the last view lets go."""
+    """One device allocation; returned to the pool when the last view lets
+    go."""
 
     __slots__ = ("ptr", "nbytes")
 
     def __init__(self, nbytes):
-        p = vp()
-        call("dyf_malloc", C.byref(p), C.c_size_t(max(int(nbytes), 1)))
-        self.ptr = p.value
-        self.nbytes = int(nbytes)
+        global _pool_bytes
+        size = _size_class(nbytes)
+        held = _pool.get(size)
+        if held:
+            self.ptr = held.pop()
+            _pool_bytes -= size
+        else:
+            p = vp()
+            try:
+                call("dyf_malloc", C.byref(p), C.c_size_t(size))
+            except MemoryError:
+                if not _pool_bytes:
+                    raise
+                release_pool()
+                call("dyf_malloc", C.byref(p), C.c_size_t(size))
+            self.ptr = p.value
+        self.nbytes = size
 
     def __del__(self):
+        global _pool_bytes
         if self.ptr and _lib is not None:
-            _lib.dyf_free(vp(self.ptr))
+            _pool.setdefault(self.nbytes, []).append(self.ptr)
+            _pool_bytes += self.nbytes
             self.ptr = None
 
 
