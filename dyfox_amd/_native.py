@@ -103,6 +103,10 @@ _SIGNATURES = {
     "dyf_memcpy_d2h": [vp, vp, C.c_size_t],
     "dyf_memcpy_d2d": [vp, vp, C.c_size_t],
     "dyf_memset": [vp, C.c_int, C.c_size_t],
+    "dyf_memcpy2d_d2h": [vp, C.c_size_t, vp, C.c_size_t, C.c_size_t,
+                         C.c_size_t],
+    "dyf_host_alloc": [C.POINTER(vp), C.c_size_t],
+    "dyf_host_free": [vp],
     "dyf_mem_info": [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)],
     "dyf_synchronize": [],
     "dyf_stream_sync": [vp],
@@ -343,6 +347,56 @@ def release_pool():
     _pool_bytes = 0
 
 
+# Page-locked host blocks, pooled like the device blocks: a NumPy array
+# handed out by pinned_empty keeps its block alive (as the array's base)
+# and returns it here when the last view is gone.  At most
+# _HOST_POOL_LIMIT bytes are kept; beyond that freed blocks are unpinned.
+_host_pool = {}
+_host_pool_bytes = 0
+_HOST_POOL_LIMIT = 1 << 30
+
+
+class _HostBlock:
+    __slots__ = ("ptr", "nbytes", "__weakref__")
+
+    def __init__(self, nbytes):
+        global _host_pool_bytes
+        size = _size_class(nbytes)
+        held = _host_pool.get(size)
+        if held:
+            self.ptr = held.pop()
+            _host_pool_bytes -= size
+        else:
+            p = vp()
+            call("dyf_host_alloc", C.byref(p), C.c_size_t(size))
+            self.ptr = p.value
+        self.nbytes = size
+
+    @property
+    def __array_interface__(self):
+        return {"shape": (self.nbytes,), "typestr": "|u1",
+                "data": (self.ptr, False), "version": 3}
+
+    def __del__(self):
+        global _host_pool_bytes
+        if self.ptr and _lib is not None:
+            if _host_pool_bytes + self.nbytes <= _HOST_POOL_LIMIT:
+                _host_pool.setdefault(self.nbytes, []).append(self.ptr)
+                _host_pool_bytes += self.nbytes
+            else:
+                _lib.dyf_host_free(vp(self.ptr))
+            self.ptr = None
+
+
+def pinned_empty(shape, dtype):
+    """An uninitialised C-contiguous NumPy array in page-locked memory."""
+    dtype = np.dtype(dtype)
+    shape = tuple(int(n) for n in np.atleast_1d(shape))
+    count = int(np.prod(shape, dtype=np.int64))
+    raw = np.asarray(_HostBlock(count * dtype.itemsize))
+    return raw[:count * dtype.itemsize].view(dtype).reshape(shape)
+
+
 class _Block:
     """One device allocation; returned to the pool when the last view lets
     go."""
@@ -430,12 +484,28 @@ class DeviceArray:
     def __len__(self):
         return self.shape[0]
 
-    def get(self):
-        """Host copy (NumPy)."""
-        out = np.empty(self.shape, self.dtype)
-        if self.nbytes:
+    def get(self, out=None):
+        """Host copy (NumPy), into `out` when given: a host array of the
+        same dtype and shape that is C-contiguous, or whose first axis is
+        strided over C-contiguous rows (a slice of a larger array)."""
+        if out is None:
+            out = np.empty(self.shape, self.dtype)
+        if out.dtype != self.dtype or out.shape != self.shape:
+            raise ValueError(f"out is {out.dtype} {out.shape}, the array "
+                             f"{self.dtype} {self.shape}")
+        if not self.nbytes:
+            return out
+        if out.flags.c_contiguous:
             call("dyf_memcpy_d2h", vp(out.ctypes.data), vp(self.ptr),
                  C.c_size_t(self.nbytes))
+            return out
+        row = self.nbytes // self.shape[0]
+        if out.ndim < 2 or not out[0].flags.c_contiguous:
+            raise ValueError("out must be C-contiguous or strided along its "
+                             "first axis only")
+        call("dyf_memcpy2d_d2h", vp(out.ctypes.data), C.c_size_t(out.strides[0]),
+             vp(self.ptr), C.c_size_t(row), C.c_size_t(row),
+             C.c_size_t(self.shape[0]))
         return out
 
     def __array__(self, dtype=None, copy=None):

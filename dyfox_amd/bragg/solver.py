@@ -120,7 +120,8 @@ def _workspace(nbytes):
 def gmres_batch(cfg, xtal, s_dev, E, incident, *, tol, restart, maxiter,
                 x0=None, planes=False):
     """One batched GMRES call: v (n_s, nyc, Nx) device and per-system
-    (residual, iterations, code).
+    (residual, iterations, code) records (a NumPy structured array with
+    those fields; a record unpacks as the tuple).
 
     planes=False: system j is s_dev[j] over all nyc planes of its table
     (one Krylov space per angle, the reference's chunk).  planes=True:
@@ -156,17 +157,18 @@ def gmres_batch(cfg, xtal, s_dev, E, incident, *, tol, restart, maxiter,
            C.c_void_p(E.ptr), C.c_void_p(inc.ptr), C.byref(opts),
            None if x0d is None else C.c_void_p(x0d.ptr), C.c_void_p(x.ptr),
            C.c_void_p(ws.ptr), C.c_size_t(ws.nbytes), info, None)
-    return x, [(info[j].residual, info[j].iterations, info[j].code)
-               for j in range(n)]
+    return x, np.ctypeslib.as_array(info).copy()
 
 
-def exit_wave(cfg, xtal, E, v):
-    """v e^{-iH(z=0)} on the host, (n_s, nyc, Nx)."""
+def exit_wave(cfg, xtal, E, v, out=None):
+    """v e^{-iH(z=0)} on the host, (n_s, nyc, Nx); written into `out`
+    when given (a host array of that shape, C-contiguous or a slice along
+    its first axis of one)."""
     a = bragg_args(cfg, xtal, E.shape[-2], E.ndim == 4)
-    out = N.DeviceArray.empty(v.shape, np.complex128)
+    dev = N.DeviceArray.empty(v.shape, np.complex128)
     N.call("dyf_bragg_exit_wave", C.byref(a), v.shape[0], C.c_void_p(E.ptr),
-           C.c_void_p(v.ptr), C.c_void_p(out.ptr), None)
-    return out.get()
+           C.c_void_p(v.ptr), C.c_void_p(dev.ptr), None)
+    return dev.get(out)
 
 
 def _pick(arr, idx):
@@ -206,7 +208,9 @@ def solve_surface_batch(seg_list, s_dev, cfg, y_values, xtal, g_vec,
     n_s = len(s_all)
     y_values = np.asarray(y_values, np.float64)
     ny = len(y_values)
-    out = np.zeros((n_s, ny, cfg.Nx), np.complex128)
+    # every (angle, plane) is written below; page-locked so the exit
+    # waves come back at link speed
+    out = N.pinned_empty((n_s, ny, cfg.Nx), np.complex128)
     stats = [dict(worst=0.0, its=0, retries=0, splits=0, unconverged=0)
              for _ in range(n_s)]
     inc = cfg.incident(n=1)[0]
@@ -231,12 +235,15 @@ def solve_surface_batch(seg_list, s_dev, cfg, y_values, xtal, g_vec,
         res = {}
         cnt = {s: 0 for s in pending}
         V = {}
+        first = None                 # the first attempt's solutions
         for _ in range(tries):
             v, info = gmres_batch(cfg, xtal, s_all[pending], E, inc, tol=tol,
                                   restart=r, maxiter=maxiter, x0=x0)
+            if first is None:
+                first = v
             for j, s in enumerate(pending):
-                res[s] = info[j][0]
-                cnt[s] += info[j][1]
+                res[s] = float(info[j][0])
+                cnt[s] += int(info[j][1])
                 V[s] = v[j]                     # device view
             retry = [j for j, s in enumerate(pending)
                      if res[s] > accept and r < cap]
@@ -259,16 +266,27 @@ def solve_surface_batch(seg_list, s_dev, cfg, y_values, xtal, g_vec,
                  if res[s] > accept and nyc > 1 and depth < max_depth]
         keep = [s for s in sys_idx if s not in split]
         if keep:
-            vk = N.DeviceArray.empty((len(keep), nyc, cfg.Nx), np.complex128)
-            for j, s in enumerate(keep):
-                vk[j].copy_from(V[s])
-            ew = exit_wave(cfg, xtal, E, vk)
-            for j, s in enumerate(keep):
+            whole = keep == list(sys_idx) and all(
+                V[s].ptr == first[j].ptr for j, s in enumerate(keep))
+            if whole and keep == list(range(keep[0], keep[0] + len(keep))):
+                # every system kept from the first attempt, contiguous in
+                # out: the exit waves go straight into their slice
+                exit_wave(cfg, xtal, E, first,
+                          out=out[keep[0]:keep[0] + len(keep), j0:j1])
+            else:
+                vk = first if whole else N.DeviceArray.empty(
+                    (len(keep), nyc, cfg.Nx), np.complex128)
+                if not whole:
+                    for j, s in enumerate(keep):
+                        vk[j].copy_from(V[s])
+                ew = exit_wave(cfg, xtal, E, vk)
+                for j, s in enumerate(keep):
+                    out[s, j0:j1] = ew[j]
+            for s in keep:
                 st = stats[s]
                 st["unconverged"] += res[s] > accept
                 st["worst"] = max(st["worst"], res[s])
                 st["its"] = max(st["its"], cnt[s])
-                out[s, j0:j1] = ew[j]
                 if verbose:
                     print(f"      s {s} y {j0}-{j1}: {cnt[s]} its, res "
                           f"{res[s]:.1e}", flush=True)
@@ -285,7 +303,7 @@ def solve_surface_batch(seg_list, s_dev, cfg, y_values, xtal, g_vec,
         E = chunk_phase(seg_list, cfg, xtal, g_vec, y_values[j0:j1])
         cap = max(restart, min(restart_cap, 8))
         r = max(restart, min(cap, per_plane))
-        pending = list(range(n_s))
+        pending = np.arange(n_s)
         x0 = None if warm is None else N.DeviceArray.from_host(
             np.ascontiguousarray(warm[:, j0:j1]))
         # the solutions stay on the device: only the exit waves come back
@@ -300,13 +318,10 @@ def solve_surface_batch(seg_list, s_dev, cfg, y_values, xtal, g_vec,
                 V = v
             else:
                 for j, s in enumerate(pending):
-                    V[s].copy_from(v[j])
-            for j, s in enumerate(pending):
-                for p in range(nyc):
-                    rs, it, _ = info[j * nyc + p]
-                    res[s, p] = rs
-                    cnt[s, p] += it
-            retry = [s for s in pending if res[s].max() > accept]
+                    V[int(s)].copy_from(v[j])
+            res[pending] = info["residual"].reshape(len(pending), nyc)
+            cnt[pending] += info["iterations"].reshape(len(pending), nyc)
+            retry = [int(s) for s in pending if res[s].max() > accept]
             if not retry or r >= cap:
                 break
             for s in retry:
@@ -314,15 +329,16 @@ def solve_surface_batch(seg_list, s_dev, cfg, y_values, xtal, g_vec,
             x0 = N.DeviceArray.empty((len(retry), nyc, cfg.Nx), np.complex128)
             for j, s in enumerate(retry):
                 x0[j].copy_from(V[s])
-            pending = retry
+            pending = np.asarray(retry)
             r = min(r * grow, cap)
-        ew = exit_wave(cfg, xtal, E, V)
+        exit_wave(cfg, xtal, E, V, out=out[:, j0:j1])
+        unconv = (res > accept).sum(axis=1)
+        worst, its = res.max(axis=1), cnt.max(axis=1)
         for s in range(n_s):
             st = stats[s]
-            st["unconverged"] += int((res[s] > accept).sum())
-            st["worst"] = max(st["worst"], float(res[s].max()))
-            st["its"] = max(st["its"], int(cnt[s].max()))
-            out[s, j0:j1] = ew[s]
+            st["unconverged"] += int(unconv[s])
+            st["worst"] = max(st["worst"], float(worst[s]))
+            st["its"] = max(st["its"], int(its[s]))
             if verbose:
                 print(f"      s {s} y {j0}-{j1}: {int(cnt[s].max())} its "
                       f"(plane max), res {res[s].max():.1e}", flush=True)
@@ -336,6 +352,9 @@ def solve_surface_batch(seg_list, s_dev, cfg, y_values, xtal, g_vec,
             solve_block(list(range(n_s)), j0, min(j0 + chunk, ny))
         else:
             solve_planes(j0, min(j0 + chunk, ny))
+    # solve_block names itself through its closure; clearing the names
+    # breaks that cycle, so `out` is not held until a garbage collection
+    solve_block = solve_planes = None
     infos = [{"residual": st["worst"], "iterations": st["its"],
               "retries": st["retries"], "splits": st["splits"],
               "unconverged_chunks": st["unconverged"],
@@ -391,5 +410,5 @@ def solve_bragg_stable(seg_list, s_dev, cfg, xtal, g_vec, xp=None,
     v, info = gmres_batch(cfg, xtal, [float(s_dev)], E, inc, tol=tol,
                           restart=restart, maxiter=maxiter)
     ew = exit_wave(cfg, xtal, E, v)[0, 0]
-    res, its, code = info[0]
+    res, its, code = float(info[0][0]), int(info[0][1]), int(info[0][2])
     return ew, {"gmres_code": code, "iterations": its, "residual": res}

@@ -496,6 +496,26 @@ __global__ void arnoldi_finish_kernel(int nsys, int restart, int col, cd *H,
 
 __global__ void zero_int_kernel(int *p) { *p = 0; }
 
+// Start of a restart cycle: inv_s = 1/||M r_s||, S_s = [||M r_s||, 0, ...]
+// (the first column of the Arnoldi basis is M r_s * inv_s).
+__global__ void cycle_start_kernel(int nsys, int R, const cd *nrm,
+                                   double *inv, cd *S) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (size_t)nsys * (R + 1)) return;
+    const int s = (int)(i / (R + 1)), q = (int)(i % (R + 1));
+    const double t = sqrt(nrm[s].r);
+    S[i] = q == 0 ? cd{t, 0.0} : cd{0.0, 0.0};
+    if (q == 0) inv[s] = t > 0.0 ? 1.0 / t : 0.0;
+}
+
+// After the inner loop (which clears a system's flag as its Arnoldi
+// process ends): every system not yet done takes the update again.
+__global__ void restore_active_kernel(int nsys, const SysState *st,
+                                      int *active) {
+    const int s = blockIdx.x * 64 + threadIdx.x;
+    if (s < nsys) active[s] = !st[s].done;
+}
+
 // Upper-triangular solve of the Hessenberg least squares (SciPy's loop),
 // y stored per system; ncol = col + 1 columns.
 __global__ void lsq_kernel(int nsys, int restart, const cd *H, cd *S, cd *Y,
@@ -897,34 +917,30 @@ extern "C" int dyf_bragg_solve(const dyf_bragg_args *a, const double *s_dev,
 
     std::vector<SysState> st(nsys);
     std::vector<int> act(nsys, 1);
-    // Small host<->device state copies are ordered on the stream and
-    // completed before returning, so the host vectors can change freely.
+    // Host<->device traffic is a few small copies per restart cycle, each
+    // group followed by one synchronisation; the host vectors are only
+    // written after such a synchronisation, so the asynchronous uploads
+    // never see them change.
     auto push_state = [&]() -> int {
         DYF_HIP(hipMemcpyAsync(W.st, st.data(), sizeof(SysState) * nsys,
                                hipMemcpyHostToDevice, stream));
         DYF_HIP(hipMemcpyAsync(W.active, act.data(), sizeof(int) * nsys,
                                hipMemcpyHostToDevice, stream));
-        DYF_HIP(hipStreamSynchronize(stream));
         return DYF_OK;
     };
-    auto pull_state = [&]() -> int {
-        DYF_HIP(hipMemcpyAsync(st.data(), W.st, sizeof(SysState) * nsys,
+    auto norm_into = [&](const cd *v, cd *out, const int *active) -> int {
+        return dots(v, N, 0, 1, v, N, N, out, 1, W.part, nsys, active,
+                    stream);
+    };
+    auto fetch = [&](std::vector<cd> &h, const cd *d) -> int {
+        DYF_HIP(hipMemcpyAsync(h.data(), d, sizeof(cd) * nsys,
                                hipMemcpyDeviceToHost, stream));
-        DYF_HIP(hipStreamSynchronize(stream));
         return DYF_OK;
     };
-    std::vector<cd> nrm(nsys);
-    auto norms = [&](const cd *v, size_t sys_stride, const int *active) -> int {
-        int r2 = dots(v, sys_stride, 0, 1, v, sys_stride, N, W.nrm, 1, W.part,
-                      nsys, active, stream);
-        if (r2) return r2;
-        DYF_HIP(hipMemcpyAsync(nrm.data(), W.nrm, sizeof(cd) * nsys,
-                               hipMemcpyDeviceToHost, stream));
-        DYF_HIP(hipStreamSynchronize(stream));
-        return DYF_OK;
-    };
+    std::vector<cd> nb(nsys), nmb(nsys), nr(nsys);
 
-    // preconditioner symbol for every system; b = -march(incident, 0)
+    // preconditioner symbol for every system; b = -march(incident, 0);
+    // ||b||, ||M b|| and ||r|| (r = b - A x0, or b) in one round trip
     bragg_symbol_kernel<<<nsys, 256, 0, stream>>>(k, s_dev, W.pinv);
     DYF_LAUNCHED("bragg_symbol_kernel");
     rc = launch_march(k, s_dev, E, (const cd *)incident, nullptr, 0, W.b, N,
@@ -937,48 +953,59 @@ extern "C" int dyf_bragg_solve(const dyf_bragg_args *a, const double *s_dev,
         DYF_HIP(hipMemsetAsync(x, 0, sizeof(cd) * nsys * N, stream));
     }
     DYF_HIP(hipMemsetAsync(W.H, 0, sizeof(cd) * nsys * R * (R + 1), stream));
-    // ||b|| and ||M b||
-    rc = norms(W.b, N, nullptr);
-    if (rc) return rc;
-    for (int s = 0; s < nsys; ++s) {
-        st[s] = SysState{};
-        st[s].bnrm2 = sqrt(nrm[s].r);
-        st[s].atol = opt->tol * st[s].bnrm2;
-        st[s].pmf = 1.0;
-        st[s].code = opt->maxiter;
-        if (st[s].bnrm2 == 0.0) {   // SciPy returns b (= 0) at once
-            st[s].done = 1, st[s].code = 0, act[s] = 0;
-        }
-    }
-    rc = apply_precon(k, W.pinv, W.b, W.w, stream);
-    if (rc) return rc;
-    rc = norms(W.w, N, nullptr);
-    if (rc) return rc;
-    for (int s = 0; s < nsys; ++s) {
-        const double Mb = sqrt(nrm[s].r);
-        st[s].ptol = st[s].bnrm2 > 0.0
-                         ? Mb * std::min(1.0, st[s].atol / st[s].bnrm2)
-                         : 0.0;
-    }
-    // r = b - A x (or b when x = 0)
+    if ((rc = norm_into(W.b, W.nrm, nullptr))) return rc;
+    if ((rc = apply_precon(k, W.pinv, W.b, W.w, stream))) return rc;
+    if ((rc = norm_into(W.w, W.nrm0, nullptr))) return rc;
     if (x0) {
         rc = launch_march(k, s_dev, E, nullptr, x, N, W.av, N, 1.0, nullptr,
                           stream);
         if (rc) return rc;
         sub_kernel<<<vgrid, 256, 0, stream>>>(W.b, W.av, W.r, N, nullptr);
         DYF_LAUNCHED("sub_kernel");
+        if ((rc = norm_into(W.r, W.h2, nullptr))) return rc;
+        if ((rc = fetch(nr, W.h2))) return rc;
     } else {
         DYF_HIP(hipMemcpyAsync(W.r, W.b, sizeof(cd) * nsys * N,
                                hipMemcpyDeviceToDevice, stream));
     }
-    rc = norms(W.r, N, nullptr);
-    if (rc) return rc;
+    if ((rc = fetch(nb, W.nrm)) || (rc = fetch(nmb, W.nrm0))) return rc;
+    DYF_HIP(hipStreamSynchronize(stream));
+    if (!x0) nr = nb;
     for (int s = 0; s < nsys; ++s) {
-        st[s].rnorm = sqrt(nrm[s].r);
+        st[s] = SysState{};
+        st[s].bnrm2 = sqrt(nb[s].r);
+        st[s].atol = opt->tol * st[s].bnrm2;
+        st[s].pmf = 1.0;
+        st[s].code = opt->maxiter;
+        if (st[s].bnrm2 == 0.0) {   // SciPy returns b (= 0) at once
+            st[s].done = 1, st[s].code = 0, act[s] = 0;
+        }
+        const double Mb = sqrt(nmb[s].r);
+        st[s].ptol = st[s].bnrm2 > 0.0
+                         ? Mb * std::min(1.0, st[s].atol / st[s].bnrm2)
+                         : 0.0;
+        st[s].rnorm = sqrt(nr[s].r);
         if (!st[s].done && st[s].rnorm < st[s].atol) {
             st[s].done = 1, st[s].code = 0, act[s] = 0;
         }
     }
+
+    // The inner loop reads each step's "still iterating" word one step
+    // late: the copy of step c lands in a page-locked slot with an event,
+    // and the host waits on step c-1's event only after enqueueing step
+    // c, so the device never idles on the host.  A step enqueued after
+    // every system stopped is masked out by the active flags (only the
+    // preconditioner's FFTs run on it).
+    static thread_local int *still_h = nullptr;
+    static thread_local hipEvent_t still_ev[2] = {nullptr, nullptr};
+    if (!still_h) {
+        DYF_HIP(hipHostMalloc((void **)&still_h, 2 * sizeof(int),
+                              hipHostMallocDefault));
+        for (int e = 0; e < 2; ++e)
+            DYF_HIP(hipEventCreateWithFlags(&still_ev[e],
+                                            hipEventDisableTiming));
+    }
+    const unsigned sgrid = (unsigned)(((size_t)nsys * (R + 1) + 255) / 256);
 
     for (int cycle = 0; cycle < opt->maxiter; ++cycle) {
         int any = 0;
@@ -988,25 +1015,13 @@ extern "C" int dyf_bragg_solve(const dyf_bragg_args *a, const double *s_dev,
             any |= act[s];
         }
         if (!any) break;
-        rc = push_state();
-        if (rc) return rc;
+        if ((rc = push_state())) return rc;
         // v0 = M r / ||M r||, S = [||M r||, 0, ...]
-        rc = apply_precon(k, W.pinv, W.r, W.w, stream);
-        if (rc) return rc;
-        rc = norms(W.w, N, W.active);
-        if (rc) return rc;
-        std::vector<double> inv(nsys, 0.0);
-        std::vector<cd> S0((size_t)nsys * (R + 1), cd{0.0, 0.0});
-        for (int s = 0; s < nsys; ++s) {
-            const double t = sqrt(nrm[s].r);
-            inv[s] = t > 0.0 ? 1.0 / t : 0.0;
-            S0[(size_t)s * (R + 1)] = cd{t, 0.0};
-        }
-        DYF_HIP(hipMemcpyAsync(W.inv, inv.data(), sizeof(double) * nsys,
-                               hipMemcpyHostToDevice, stream));
-        DYF_HIP(hipMemcpyAsync(W.S, S0.data(), sizeof(cd) * S0.size(),
-                               hipMemcpyHostToDevice, stream));
-        DYF_HIP(hipStreamSynchronize(stream));
+        if ((rc = apply_precon(k, W.pinv, W.r, W.w, stream))) return rc;
+        if ((rc = norm_into(W.w, W.nrm, W.active))) return rc;
+        cycle_start_kernel<<<sgrid, 256, 0, stream>>>(nsys, R, W.nrm, W.inv,
+                                                      W.S);
+        DYF_LAUNCHED("cycle_start_kernel");
         scale_copy_kernel<<<vgrid, 256, 0, stream>>>(W.w, N, W.V, Vsys, W.inv,
                                                      N, W.active);
         DYF_LAUNCHED("scale_copy_kernel");
@@ -1046,18 +1061,19 @@ extern "C" int dyf_bragg_solve(const dyf_bragg_args *a, const double *s_dev,
                 W.w, N, W.V + (size_t)(col + 1) * N, Vsys, W.inv, N,
                 W.active);
             DYF_LAUNCHED("scale_copy_kernel");
-            int still = 0;
-            DYF_HIP(hipMemcpyAsync(&still, W.still, sizeof(int),
+            DYF_HIP(hipMemcpyAsync(&still_h[col & 1], W.still, sizeof(int),
                                    hipMemcpyDeviceToHost, stream));
-            DYF_HIP(hipStreamSynchronize(stream));
-            if (!still) break;
+            DYF_HIP(hipEventRecord(still_ev[col & 1], stream));
+            if (col > 0) {
+                DYF_HIP(hipEventSynchronize(still_ev[(col - 1) & 1]));
+                if (!still_h[(col - 1) & 1]) break;
+            }
         }
-        rc = pull_state();
-        if (rc) return rc;
-        // solve the small least-squares problems and update x
-        for (int s = 0; s < nsys; ++s) act[s] = !st[s].done;
-        rc = push_state();
-        if (rc) return rc;
+        // solve the small least-squares problems, update x and form the
+        // true residual for the systems not yet done; then one round trip
+        restore_active_kernel<<<(nsys + 63) / 64, 64, 0, stream>>>(
+            nsys, W.st, W.active);
+        DYF_LAUNCHED("restore_active_kernel");
         lsq_kernel<<<(nsys + 63) / 64, 64, 0, stream>>>(nsys, R, W.H, W.S,
                                                         W.Y, W.st, W.active);
         DYF_LAUNCHED("lsq_kernel");
@@ -1069,12 +1085,15 @@ extern "C" int dyf_bragg_solve(const dyf_bragg_args *a, const double *s_dev,
         if (rc) return rc;
         sub_kernel<<<vgrid, 256, 0, stream>>>(W.b, W.av, W.r, N, W.active);
         DYF_LAUNCHED("sub_kernel");
-        rc = norms(W.r, N, W.active);
-        if (rc) return rc;
+        if ((rc = norm_into(W.r, W.nrm, W.active))) return rc;
+        DYF_HIP(hipMemcpyAsync(st.data(), W.st, sizeof(SysState) * nsys,
+                               hipMemcpyDeviceToHost, stream));
+        if ((rc = fetch(nr, W.nrm))) return rc;
+        DYF_HIP(hipStreamSynchronize(stream));
         for (int s = 0; s < nsys; ++s) {
             if (st[s].done) continue;
             SysState &q = st[s];
-            q.rnorm = sqrt(nrm[s].r);
+            q.rnorm = sqrt(nr[s].r);
             if (q.rnorm <= q.atol) {
                 q.done = 1, q.code = 0;
             } else if (q.breakdown) {

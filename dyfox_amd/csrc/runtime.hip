@@ -186,6 +186,35 @@ int dyf_memset(void *ptr, int value, size_t bytes) {
     return DYF_OK;
 }
 
+int dyf_memcpy2d_d2h(void *dst, size_t dpitch, const void *src,
+                     size_t spitch, size_t width, size_t height) {
+    if (!width || !height) return DYF_OK;
+    if (!dst || !src) return fail(DYF_EINVAL, "NULL pointer to dyf_memcpy2d_d2h");
+    if (dpitch < width || spitch < width)
+        return fail(DYF_EINVAL, "pitch below the row width");
+    DYF_HIP(hipMemcpy2D(dst, dpitch, src, spitch, width, height,
+                        hipMemcpyDeviceToHost));
+    return DYF_OK;
+}
+
+int dyf_host_alloc(void **ptr, size_t bytes) {
+    if (!ptr) return fail(DYF_EINVAL, "ptr is NULL");
+    *ptr = nullptr;
+    hipError_t e = hipHostMalloc(ptr, bytes ? bytes : 256, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        *ptr = nullptr;
+        (void)hipGetLastError();
+        return fail(DYF_ENOMEM, "hipHostMalloc(%zu bytes): %s", bytes,
+                    hipGetErrorString(e));
+    }
+    return DYF_OK;
+}
+
+int dyf_host_free(void *ptr) {
+    if (ptr) DYF_HIP(hipHostFree(ptr));
+    return DYF_OK;
+}
+
 int dyf_mem_info(size_t *free_bytes, size_t *total_bytes) {
     size_t f = 0, t = 0;
     DYF_HIP(hipMemGetInfo(&f, &t));

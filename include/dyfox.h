@@ -59,6 +59,16 @@ int dyf_memcpy_h2d(void *dst_dev, const void *src, size_t bytes);
 int dyf_memcpy_d2h(void *dst, const void *src_dev, size_t bytes);
 int dyf_memcpy_d2d(void *dst_dev, const void *src_dev, size_t bytes);
 int dyf_memset(void *ptr_dev, int value, size_t bytes);
+/* Strided device-to-host copy: `height` rows of `width` bytes, rows
+ * `spitch` bytes apart on the device and `dpitch` bytes apart on the host
+ * (a chunk of planes into its slice of a larger host array).             */
+int dyf_memcpy2d_d2h(void *dst, size_t dpitch, const void *src_dev,
+                     size_t spitch, size_t width, size_t height);
+/* Page-locked host memory (hipHostMalloc): device-to-host copies into it
+ * run at link speed instead of through a pageable staging path (CuPy's
+ * pinned memory pool plays this part for the reference).                 */
+int dyf_host_alloc(void **ptr, size_t bytes);
+int dyf_host_free(void *ptr);
 int dyf_mem_info(size_t *free_bytes, size_t *total_bytes);
 int dyf_synchronize(void);
 int dyf_stream_sync(void *stream);
