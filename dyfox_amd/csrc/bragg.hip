@@ -74,12 +74,25 @@ static BraggK make_k(const dyf_bragg_args *a, int nsys) {
 }
 
 // ---- the down-march: one workgroup per (system, plane) ----------------
-// Each thread owns source columns x; per step it forms
+// The reference's step (per source column x, periodic in x)
 //   p = D0[x] + ds sbar e^{+iH} Dg[x]  -> D0'[x+1] = e0 p
 //   q = Dg[x] - ds sh  e^{-iH} D0[x]   -> Dg'[x-1] = q / eg
-// (periodic in x), so every e^{iH} element is read once per step.
+// is marched in the interaction picture X_k = D0_k / e0^k,
+// Y_k = Dg_k eg^k, where the diagonal factors drop out of the step:
+//   X'[x+1] = X[x] + c_k e^{+iH}[x] Y[x],  c_k =  ds sbar (e0 eg)^-k
+//   Y'[x-1] = Y[x] + b_k e^{-iH}[x] X[x],  b_k = -ds sh   (e0 eg)^k
+// and Dg = eg^-(Nz-1) Y at the end (16 FP64 operations per column-step
+// instead of 28).  c_k, b_k are formed per system with exp/sincos of
+// k times the exponent (no running products) into LDS.
 constexpr int kMarchThreads = 256;
 constexpr int kMaxPerThread = 8;          // Nx <= 2048
+
+// LDS of one march workgroup: [X | Y] x 2 buffers of Nx + 1 entries, then
+// c_k and b_k for the Nz - 1 steps.
+static size_t march_lds_bytes(const BraggK &k) {
+    return ((size_t)4 * (k.Nx + 1) + (size_t)2 * std::max(k.Nz - 1, 1)) *
+           sizeof(cd);
+}
 
 // M = ceil(Nx / 256) columns per thread.  The loop is branch-free: loads
 // use a clamped index and out-of-range columns write to a spare LDS slot
@@ -97,21 +110,27 @@ __global__ __launch_bounds__(kMarchThreads) void bragg_march_kernel(
     if (active && !active[s]) return;
     E += table_offset(k, s);
     const int Nx = k.Nx, tid = threadIdx.x, P = Nx + 1;
-    // LDS: [D0 | Dg] x 2 buffers of P = Nx + 1 entries, addressed by
+    const int nst = k.Nz - 1;
+    // LDS: [X | Y] x 2 buffers of P = Nx + 1 entries, addressed by
     // integer offsets into lds_b (selecting among pointers kept in an
     // array loses the LDS address space and turns every access into a
-    // flat one, which then waits for outstanding global loads too)
+    // flat one, which then waits for outstanding global loads too), then
+    // c_k at 4P and b_k at 4P + nst
     for (int x = tid; x < Nx; x += kMarchThreads) {
         lds_b[x] = inc ? inc[x] : cd{0.0, 0.0};
         lds_b[P + x] = vin ? vin[(size_t)s * vin_stride + (size_t)j * Nx + x]
                            : cd{0.0, 0.0};
     }
     const double alpha = -2.0 * kPi * s_dev[s];
-    const cd e0 = cexp(k.s0r * k.ds, k.s0i * k.ds);
-    const cd eg = cexp(k.s0r * k.ds, (k.s0i + alpha) * k.ds);
-    const cd ieg = cdiv(cd{1.0, 0.0}, eg);
-    const cd A = {k.ds * k.sbr, k.ds * k.sbi};   // ds sigma_hbar
-    const cd B = {k.ds * k.shr, k.ds * k.shi};   // ds sigma_h
+    // e0 eg = exp(lr + i li) per step
+    const double lr = 2.0 * k.s0r * k.ds, li = (2.0 * k.s0i + alpha) * k.ds;
+    const int oc = 4 * P, ob = 4 * P + nst;
+    for (int t = tid; t < nst; t += kMarchThreads) {
+        const cd f = cexp(-(double)t * lr, -(double)t * li);   // (e0 eg)^-t
+        const cd g = cexp((double)t * lr, (double)t * li);     // (e0 eg)^t
+        lds_b[oc + t] = cmul(cd{k.ds * k.sbr, k.ds * k.sbi}, f);
+        lds_b[ob + t] = cmul(cd{-k.ds * k.shr, -k.ds * k.shi}, g);
+    }
     // per column m: the source column (clamped for loads and LDS reads)
     // and the destinations of the shifted fields (slot Nx when unused)
     int xs[M], to0[M], tog[M];
@@ -130,39 +149,153 @@ __global__ __launch_bounds__(kMarchThreads) void bragg_march_kernel(
     cd en[M];
 #pragma unroll
     for (int m = 0; m < M; ++m) en[m] = Ej[xs[m]];
-    for (int step = 0; step < k.Nz - 1; ++step) {
+    for (int step = 0; step < nst; ++step) {
         cd ec[M];
 #pragma unroll
         for (int m = 0; m < M; ++m) ec[m] = en[m];
-        const int sn = step + 1 < k.Nz - 1 ? step + 1 : step;
+        const int sn = step + 1 < nst ? step + 1 : step;
         const cd *En = Ej + (size_t)sn * plane;
 #pragma unroll
         for (int m = 0; m < M; ++m) en[m] = En[xs[m]];
+        const cd ck = lds_b[oc + step], bk = lds_b[ob + step];
         const int o0 = cur * 2 * P, n0 = (1 - cur) * 2 * P;
 #pragma unroll
         for (int m = 0; m < M; ++m) {
-            const cd a0 = lds_b[o0 + xs[m]], ag = lds_b[o0 + P + xs[m]];
+            const cd X = lds_b[o0 + xs[m]], Y = lds_b[o0 + P + xs[m]];
             const cd e = ec[m];
-            const cd cb = cmul(A, e);          // ds sbar e^{+iH}
-            const cd ch = cconjmul(e, B);      // ds sh  e^{-iH}
-            const cd p = cadd(a0, cmul(cb, ag));
-            const cd nq = cmul(ch, a0);
-            const cd q = {ag.r - nq.r, ag.i - nq.i};
-            lds_b[n0 + to0[m]] = cmul(e0, p);
-            lds_b[n0 + P + tog[m]] = cmul(q, ieg);
+            const cd u = cmul(ck, e);              // c_k e^{+iH}
+            const cd w = cmul(bk, cd{e.r, -e.i});  // b_k e^{-iH}
+            lds_b[n0 + to0[m]] = cd{fma(u.r, Y.r, fma(-u.i, Y.i, X.r)),
+                                    fma(u.r, Y.i, fma(u.i, Y.r, X.i))};
+            lds_b[n0 + P + tog[m]] = cd{fma(w.r, X.r, fma(-w.i, X.i, Y.r)),
+                                        fma(w.r, X.i, fma(w.i, X.r, Y.i))};
         }
         lds_barrier();
         cur = 1 - cur;
     }
+    // Dg = eg^-(Nz-1) Y, times the caller's scale
+    const double egr = k.s0r * k.ds, egi = (k.s0i + alpha) * k.ds;
+    const cd fin = cscale(cexp(-(double)nst * egr, -(double)nst * egi),
+                          out_scale);
     const int og = cur * 2 * P + P;
     for (int x = tid; x < Nx; x += kMarchThreads)
         out[(size_t)s * out_stride + (size_t)j * Nx + x] =
-            cscale(lds_b[og + x], out_scale);
+            cmul(lds_b[og + x], fin);
+}
+
+// ---- the same march, one wavefront per (system, plane) ----------------
+// For Nx = 64-or-fewer lanes times C with C | Nx: lane l keeps columns
+// [lC, lC + C) of X and Y in registers (Lu = Nx / C lanes; the rest idle).
+// A step is C column updates in registers, a rename by one register, and
+// one ds_bpermute per field for the column crossing a lane boundary; the
+// periodic wrap is the same bpermute with lane 0 reading lane Lu - 1 (X)
+// and lane Lu - 1 reading lane 0 (Y), because Nx - 1 is the last register
+// of lane Lu - 1.  No LDS buffers and no barrier per step (the
+// workgroup-per-plane kernel above spends most of a step on its LDS
+// round trip and barrier).  c_k, b_k are formed as there, into LDS.
+__device__ __forceinline__ cd wave_fetch(cd v, int src4) {
+    const long long r = __double_as_longlong(v.r), i = __double_as_longlong(v.i);
+    const int rl = __builtin_amdgcn_ds_bpermute(src4, (int)r);
+    const int rh = __builtin_amdgcn_ds_bpermute(src4, (int)(r >> 32));
+    const int il = __builtin_amdgcn_ds_bpermute(src4, (int)i);
+    const int ih = __builtin_amdgcn_ds_bpermute(src4, (int)(i >> 32));
+    return {__longlong_as_double(((long long)rh << 32) | (unsigned)rl),
+            __longlong_as_double(((long long)ih << 32) | (unsigned)il)};
+}
+
+template <int C>
+__global__ __launch_bounds__(kWave) void bragg_march_wave_kernel(
+    BraggK k, const double *__restrict__ s_dev, const cd *__restrict__ E,
+    const cd *__restrict__ inc, const cd *__restrict__ vin, size_t vin_stride,
+    cd *__restrict__ out, size_t out_stride, double out_scale,
+    const int *__restrict__ active) {
+    extern __shared__ cd coef[];                 // c_k at 0, b_k at nst
+    const int s = blockIdx.x / k.nyc, j = blockIdx.x % k.nyc;
+    if (active && !active[s]) return;
+    E += table_offset(k, s);
+    const int Nx = k.Nx, lane = threadIdx.x, Lu = Nx / C;
+    const int nst = k.Nz - 1;
+    const double alpha = -2.0 * kPi * s_dev[s];
+    const double lr = 2.0 * k.s0r * k.ds, li = (2.0 * k.s0i + alpha) * k.ds;
+    for (int t = lane; t < nst; t += kWave) {
+        const cd f = cexp(-(double)t * lr, -(double)t * li);   // (e0 eg)^-t
+        const cd g = cexp((double)t * lr, (double)t * li);     // (e0 eg)^t
+        coef[t] = cmul(cd{k.ds * k.sbr, k.ds * k.sbi}, f);
+        coef[nst + t] = cmul(cd{-k.ds * k.shr, -k.ds * k.shi}, g);
+    }
+    const bool real = lane < Lu;
+    const int x0 = (real ? lane : Lu - 1) * C;   // idle lanes read in bounds
+    const int srcX = (real ? (lane == 0 ? Lu - 1 : lane - 1) : lane) << 2;
+    const int srcY = (real ? (lane == Lu - 1 ? 0 : lane + 1) : lane) << 2;
+    cd X[C], Y[C], en[C];
+    const cd *vj = vin ? vin + (size_t)s * vin_stride + (size_t)j * Nx : nullptr;
+#pragma unroll
+    for (int i = 0; i < C; ++i) {
+        X[i] = inc ? inc[x0 + i] : cd{0.0, 0.0};
+        Y[i] = vj ? vj[x0 + i] : cd{0.0, 0.0};
+    }
+    const size_t plane = (size_t)k.tpl * Nx;
+    const cd *Ej = E + (size_t)j * Nx + x0;
+#pragma unroll
+    for (int i = 0; i < C; ++i) en[i] = Ej[i];
+    __syncthreads();
+    for (int step = 0; step < nst; ++step) {
+        cd ec[C];
+#pragma unroll
+        for (int i = 0; i < C; ++i) ec[i] = en[i];
+        const int sn = step + 1 < nst ? step + 1 : step;
+        const cd *En = Ej + (size_t)sn * plane;
+#pragma unroll
+        for (int i = 0; i < C; ++i) en[i] = En[i];
+        const cd ck = coef[step], bk = coef[nst + step];
+        cd xn[C], yn[C];
+#pragma unroll
+        for (int i = 0; i < C; ++i) {
+            const cd e = ec[i];
+            const cd u = cmul(ck, e);              // c_k e^{+iH}
+            const cd w = cmul(bk, cd{e.r, -e.i});  // b_k e^{-iH}
+            xn[i] = cd{fma(u.r, Y[i].r, fma(-u.i, Y[i].i, X[i].r)),
+                       fma(u.r, Y[i].i, fma(u.i, Y[i].r, X[i].i))};
+            yn[i] = cd{fma(w.r, X[i].r, fma(-w.i, X[i].i, Y[i].r)),
+                       fma(w.r, X[i].i, fma(w.i, X[i].r, Y[i].i))};
+        }
+        X[0] = wave_fetch(xn[C - 1], srcX);
+#pragma unroll
+        for (int i = 1; i < C; ++i) X[i] = xn[i - 1];
+        Y[C - 1] = wave_fetch(yn[0], srcY);
+#pragma unroll
+        for (int i = 0; i < C - 1; ++i) Y[i] = yn[i + 1];
+    }
+    const double egr = k.s0r * k.ds, egi = (k.s0i + alpha) * k.ds;
+    const cd fin = cscale(cexp(-(double)nst * egr, -(double)nst * egi),
+                          out_scale);
+    if (real) {
+        cd *o = out + (size_t)s * out_stride + (size_t)j * Nx + x0;
+#pragma unroll
+        for (int i = 0; i < C; ++i) o[i] = cmul(Y[i], fin);
+    }
 }
 
 using MarchFn = void (*)(BraggK, const double *, const cd *, const cd *,
                          const cd *, size_t, cd *, size_t, double,
                          const int *);
+
+constexpr int kWaveMaxC = 16;
+
+template <int... Cs>
+static MarchFn wave_fn(int C) {
+    MarchFn f = nullptr;
+    ((C == Cs ? (f = bragg_march_wave_kernel<Cs>, 0) : 0), ...);
+    return f;
+}
+
+// Columns per lane of the wavefront march for Nx: the smallest C with
+// C | Nx and Nx / C <= 64, if one is at most kWaveMaxC; else 0.
+static int wave_columns(int Nx) {
+    for (int c = (Nx + kWave - 1) / kWave; c <= kWaveMaxC; ++c)
+        if (c >= 1 && Nx % c == 0) return c;
+    return 0;
+}
 
 template <int... Ms>
 static MarchFn march_fn(int M) {
@@ -175,7 +308,18 @@ static int launch_march(const BraggK &k, const double *s_dev, const cd *E,
                         const cd *inc, const cd *vin, size_t vin_stride,
                         cd *out, size_t out_stride, double out_scale,
                         const int *active, hipStream_t stream) {
-    const size_t lds = (size_t)4 * (k.Nx + 1) * sizeof(cd);
+    const int C = wave_columns(k.Nx);
+    const size_t coef = (size_t)2 * std::max(k.Nz - 1, 1) * sizeof(cd);
+    if (C && coef <= 64 * 1024) {
+        MarchFn wf = wave_fn<1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15,
+                             16>(C);
+        hipLaunchKernelGGL(wf, dim3(k.nsys * k.nyc), dim3(kWave), coef,
+                           stream, k, s_dev, E, inc, vin, vin_stride, out,
+                           out_stride, out_scale, active);
+        DYF_LAUNCHED("bragg_march_wave_kernel");
+        return DYF_OK;
+    }
+    const size_t lds = march_lds_bytes(k);
     const int M = (k.Nx + kMarchThreads - 1) / kMarchThreads;
     MarchFn fn = march_fn<1, 2, 3, 4, 5, 6, 7, 8>(M);
     if (!fn) return fail(DYF_EINVAL, "Nx=%d exceeds %d", k.Nx,

@@ -30,8 +30,14 @@ def _rel(a, b):
                  / np.abs(np.asarray(b)).max())
 
 
-def test_phase_march_and_preconditioner_match_oracle(dev, diamond400, g, rng):
-    cfg = BraggConfig(diamond400, t_crystal=6e-6, dx=0.2e-6, half_x=2e-6)
+# Nx = 324 = 54 lanes x 6 columns takes the wavefront march; Nx = 334 =
+# 2 x 167 has no column count <= 16 that divides it into <= 64 lanes and
+# takes the workgroup-per-plane march.
+@pytest.mark.parametrize("half_x", [2e-6, 3e-6],
+                         ids=["wavefront-Nx324", "workgroup-Nx334"])
+def test_phase_march_and_preconditioner_match_oracle(dev, diamond400, g, rng,
+                                                     half_x):
+    cfg = BraggConfig(diamond400, t_crystal=6e-6, dx=0.2e-6, half_x=half_x)
     seg = threading_segment("Edge", +1, diamond400.b_mag, float(g[2]))
     ys = np.array([-0.3e-6, 0.1e-6, 0.5e-6])
     E = BS.chunk_phase(seg, cfg, diamond400, g, ys)
