@@ -44,6 +44,10 @@ def parse():
                     help="skip the CPU oracle baseline")
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the batched-ensembles leg")
+    ap.add_argument("--probe-bragg", action="store_true",
+                    help="run only cfg1 Case B batches (independent planes, "
+                         "one warm-up and five timed), for a kernel trace, "
+                         "and exit")
     ap.add_argument("--probe", action="store_true",
                     help="for counter runs: one phase build and one march "
                          "per precision, then a 1 GiB copy kernel (the "
@@ -292,10 +296,24 @@ def probe():
     print("probe done", flush=True)
 
 
+def probe_bragg():
+    """cfg1 Case B, independent planes: one warm-up batch, five timed."""
+    from dyfox_amd.backend import device
+    from dyfox_amd.crystal import DIAMOND, CrystalParams
+    device()
+    xtal = CrystalParams.from_reflection(DIAMOND, (4, 0, 0), 17.0,
+                                         f_prime=0.003, f_dprime=0.002)
+    r = time_bragg(xtal, "B", repeats=5)
+    print(f"probe-bragg done: {r['ms_per_batch']:.2f} ms per batch, "
+          f"iterations {r['gmres_iterations']}", flush=True)
+
+
 def main():
     args = parse()
     if args.probe:
         return probe()
+    if args.probe_bragg:
+        return probe_bragg()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(rows x s_dev shard with no communication; see DESIGN.md)",

@@ -920,6 +920,162 @@ __global__ __launch_bounds__(kFpThreads) void bragg_fixed_point_kernel(
     if (tid == 0) info[blockIdx.x] = dyf_fixed_info{res, iters, conv, };
 }
 
+// Wavefront form of the same solve, for Nx = Lu C with C | Nx, Lu <= 64
+// (the march's wavefront layout): lane l keeps columns [lC, lC + C) of the
+// field being marched in registers, the one-column roll is a register
+// rename plus one ds_bpermute (lane 0 reads lane Lu - 1), and there is no
+// barrier.  The stored fields are lane-interleaved per depth row
+// (element (z, i, lane) at (z C + i) 64 + lane), so every row access is
+// one contiguous 1 KB wave access per register.  The convergence test
+// reads the previous sweep's Dg(0) from memory just before it is
+// overwritten, so it keeps no copy in registers.
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+template <int C>
+__global__ __launch_bounds__(kWave) void bragg_fixed_point_wave_kernel(
+    BraggK k, const double *__restrict__ s_dev, const cd *__restrict__ E,
+    const cd *__restrict__ inc, int n_iter, double tol, cd *__restrict__ D0,
+    cd *__restrict__ Dg, cd *__restrict__ out, dyf_fixed_info *__restrict__ info) {
+    const int sys = blockIdx.x / k.nyc, j = blockIdx.x % k.nyc;
+    const int Nx = k.Nx, Nz = k.Nz, lane = threadIdx.x, Lu = Nx / C;
+    const bool real = lane < Lu;
+    const int x0 = (real ? lane : Lu - 1) * C;   // idle lanes read in bounds
+    const int src = (real ? (lane == 0 ? Lu - 1 : lane - 1) : lane) << 2;
+    const cd *Es = E + table_offset(k, sys) + (size_t)j * Nx + x0;
+    const size_t zs = (size_t)k.tpl * Nx;             // e^{iH} depth stride
+    const size_t row = (size_t)C * kWave;
+    cd *d0 = D0 + (size_t)blockIdx.x * Nz * row + lane;
+    cd *dg = Dg + (size_t)blockIdx.x * Nz * row + lane;
+    const double alpha = -2.0 * kPi * s_dev[sys];
+    const cd e0 = cexp(k.s0r * k.ds, k.s0i * k.ds);
+    const cd eg = cexp(k.s0r * k.ds, (k.s0i + alpha) * k.ds);
+    const cd sb = {k.sbr, k.sbi}, sh = {k.shr, k.shi};
+    const double hds = 0.5 * k.ds;
+
+#pragma unroll
+    for (int i = 0; i < C; ++i)
+        d0[i * kWave] = inc ? inc[x0 + i] : cd{1.0, 0.0};
+    for (int z = 0; z < Nz; ++z)
+#pragma unroll
+        for (int i = 0; i < C; ++i) dg[(size_t)z * row + i * kWave] = cd{0.0, 0.0};
+    double res = __builtin_nan("");
+    int iters = n_iter, conv = 0;
+    // The e^{iH} and frozen-field values of depth z+1 are loaded during
+    // step z-1 (index clamped at the last row, so the loop has no branch)
+    // and used in step z: a whole step covers their latency.
+    cd cur[C], cb[C], en[C], fn[C];
+    for (int it = 0; it < n_iter; ++it) {
+        // forward: D0[z+1] = e0 roll(D0[z] + hds Cb[z] Dg[z]) + hds Cb[z+1] Dg[z+1]
+#pragma unroll
+        for (int i = 0; i < C; ++i) {
+            cur[i] = d0[i * kWave];
+            cb[i] = cscale(cmul(cmul(sb, Es[i]), dg[i * kWave]), hds);
+            en[i] = Es[zs + i];
+            fn[i] = dg[row + i * kWave];
+        }
+        for (int z = 0; z < Nz - 1; ++z) {
+            const int zp = z + 2 < Nz ? z + 2 : Nz - 1;
+            const cd *En = Es + (size_t)zp * zs;
+            const cd *Gn = dg + (size_t)zp * row;
+#pragma unroll
+            for (int i = 0; i < C; ++i) {
+                cur[i] = cmul(e0, cadd(cur[i], cb[i]));
+                cb[i] = cscale(cmul(cmul(sb, en[i]), fn[i]), hds);
+            }
+            // keep the next loads below the last use of en, fn (hoisted
+            // above it they need a second register set and spill)
+            asm volatile("" ::: "memory");
+#pragma unroll
+            for (int i = 0; i < C; ++i) {
+                en[i] = En[i];
+                fn[i] = Gn[i * kWave];
+            }
+            const cd w = wave_fetch(cur[C - 1], src);
+#pragma unroll
+            for (int i = C - 1; i > 0; --i) cur[i] = cadd(cur[i - 1], cb[i]);
+            cur[0] = cadd(w, cb[0]);
+            cd *Dn = d0 + (size_t)(z + 1) * row;
+#pragma unroll
+            for (int i = 0; i < C; ++i) Dn[i * kWave] = cur[i];
+        }
+        // backward: Dg[z] = eg roll(Dg[z+1] + hds Ch[z+1] D0[z+1]) + hds Ch[z] D0[z]
+        const cd *Ez = Es + (size_t)(Nz - 1) * zs;
+#pragma unroll
+        for (int i = 0; i < C; ++i) {
+            dg[(size_t)(Nz - 1) * row + i * kWave] = cd{0.0, 0.0};
+            cb[i] = cscale(cmul(cconjmul(Ez[i], sh), cur[i]), hds);
+            cur[i] = cd{0.0, 0.0};
+            en[i] = Es[(size_t)(Nz - 2) * zs + i];
+            fn[i] = d0[(size_t)(Nz - 2) * row + i * kWave];
+        }
+        // the step z (z = 0 last, separately: its convergence test would
+        // otherwise be if-converted into every step)
+        auto back_step = [&](int z) {
+            const int zp = z > 0 ? z - 1 : 0;
+            const cd *En = Es + (size_t)zp * zs;
+            const cd *Fn = d0 + (size_t)zp * row;
+#pragma unroll
+            for (int i = 0; i < C; ++i) {
+                cur[i] = cmul(eg, cadd(cur[i], cb[i]));
+                cb[i] = cscale(cmul(cconjmul(en[i], sh), fn[i]), hds);
+            }
+            asm volatile("" ::: "memory");
+#pragma unroll
+            for (int i = 0; i < C; ++i) {
+                en[i] = En[i];
+                fn[i] = Fn[i * kWave];
+            }
+            const cd w = wave_fetch(cur[C - 1], src);
+#pragma unroll
+            for (int i = C - 1; i > 0; --i) cur[i] = cadd(cur[i - 1], cb[i]);
+            cur[0] = cadd(w, cb[0]);
+        };
+        for (int z = Nz - 2; z >= 1; --z) {
+            back_step(z);
+            cd *Dn = dg + (size_t)z * row;
+#pragma unroll
+            for (int i = 0; i < C; ++i) Dn[i * kWave] = cur[i];
+        }
+        back_step(0);
+        // convergence on the surface value Dg[0] against the last sweep's
+        double dmax = 0.0, amax = 0.0;
+#pragma unroll
+        for (int i = 0; i < C; ++i) {
+            const cd pv = dg[i * kWave];
+            const double dr = cur[i].r - pv.r, di = cur[i].i - pv.i;
+            dmax = fmax(dmax, sqrt(dr * dr + di * di));
+            amax = fmax(amax, sqrt(cur[i].r * cur[i].r + cur[i].i * cur[i].i));
+            dg[i * kWave] = cur[i];
+        }
+        dmax = wave_max(real ? dmax : 0.0);
+        amax = wave_max(real ? amax : 0.0);
+        if (it > 0) {
+            res = dmax / (amax + 1e-300);
+            if (res < tol) {
+                conv = 1, iters = it + 1;
+                break;
+            }
+        }
+    }
+    if (real) {
+#pragma unroll
+        for (int i = 0; i < C; ++i)
+            out[(size_t)blockIdx.x * Nx + x0 + i] = cconjmul(Es[i], cur[i]);
+    }
+    if (lane == 0) info[blockIdx.x] = dyf_fixed_info{res, iters, conv, };
+}
+
+template <int... Cs>
+static auto fixed_wave_fn(int C) {
+    decltype(&bragg_fixed_point_wave_kernel<1>) f = nullptr;
+    ((C == Cs ? (f = bragg_fixed_point_wave_kernel<Cs>, 0) : 0), ...);
+    return f;
+}
+
 }  // namespace dyf
 
 using namespace dyf;
@@ -989,13 +1145,21 @@ extern "C" int dyf_bragg_exit_wave(const dyf_bragg_args *a, int nsys,
     return DYF_OK;
 }
 
+// Entries per stored depth row of the fixed-point fields: the wavefront
+// kernel's lane-interleaved row (64 C) when Nx takes it, else Nx.
+static size_t fixed_row(int Nx) {
+    const int C = wave_columns(Nx);
+    return C ? (size_t)C * kWave : (size_t)Nx;
+}
+
 extern "C" int dyf_bragg_fixed_point_bytes(const dyf_bragg_args *a, int nsys,
                                            size_t *bytes) {
     int rc = check_bragg(a, nsys);
     if (rc) return rc;
     if (!bytes) return fail(DYF_EINVAL, "bytes is NULL");
     const size_t n = (size_t)nsys * a->nyc;
-    *bytes = 2 * n * a->Nz * a->Nx * sizeof(cd) + n * sizeof(dyf_fixed_info);
+    *bytes = 2 * n * a->Nz * fixed_row(a->Nx) * sizeof(cd) +
+             n * sizeof(dyf_fixed_info);
     return DYF_OK;
 }
 
@@ -1018,16 +1182,27 @@ extern "C" int dyf_bragg_fixed_point(const dyf_bragg_args *a,
                     need);
     hipStream_t stream = (hipStream_t)stream_;
     const BraggK k = make_k(a, nsys);
-    const size_t n = (size_t)nsys * a->nyc, field = n * a->Nz * a->Nx;
+    const size_t n = (size_t)nsys * a->nyc,
+                 field = n * a->Nz * fixed_row(a->Nx);
     cd *D0 = (cd *)work, *Dg = D0 + field;
     dyf_fixed_info *idev = (dyf_fixed_info *)(Dg + field);
-    // two rows of at most 2048 columns: within the default 64 KB of LDS
-    const size_t lds = (size_t)2 * a->Nx * sizeof(cd);
-    hipLaunchKernelGGL(bragg_fixed_point_kernel, dim3((unsigned)n),
-                       dim3(kFpThreads), lds, stream, k, s_dev, (const cd *)E,
-                       (const cd *)incident, opt->n_iter, opt->tol, D0, Dg,
-                       (cd *)out, idev);
-    DYF_LAUNCHED("bragg_fixed_point_kernel");
+    const int C = wave_columns(a->Nx);
+    if (C) {
+        auto wf = fixed_wave_fn<1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14,
+                                15, 16>(C);
+        hipLaunchKernelGGL(wf, dim3((unsigned)n), dim3(kWave), 0, stream, k,
+                           s_dev, (const cd *)E, (const cd *)incident,
+                           opt->n_iter, opt->tol, D0, Dg, (cd *)out, idev);
+        DYF_LAUNCHED("bragg_fixed_point_wave_kernel");
+    } else {
+        // two rows of at most 2048 columns: within the default 64 KB of LDS
+        const size_t lds = (size_t)2 * a->Nx * sizeof(cd);
+        hipLaunchKernelGGL(bragg_fixed_point_kernel, dim3((unsigned)n),
+                           dim3(kFpThreads), lds, stream, k, s_dev,
+                           (const cd *)E, (const cd *)incident, opt->n_iter,
+                           opt->tol, D0, Dg, (cd *)out, idev);
+        DYF_LAUNCHED("bragg_fixed_point_kernel");
+    }
     DYF_HIP(hipMemcpyAsync(info, idev, n * sizeof(dyf_fixed_info),
                            hipMemcpyDeviceToHost, stream));
     DYF_HIP(hipStreamSynchronize(stream));

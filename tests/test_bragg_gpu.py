@@ -256,11 +256,18 @@ def test_cfg1_case_b_matches_oracle(dev, diamond400, g):
         assert _rel(out[j], want) < 1e-8
 
 
-def test_fixed_point_matches_oracle(dev, diamond400, g):
+# Nx = 432 = 54 lanes x 8 columns takes the wavefront fixed-point kernel,
+# Nx = 452 = 4 x 113 the workgroup-per-plane one.
+FP_GRIDS = pytest.mark.parametrize("half_x", [2e-6, 3e-6],
+                                   ids=["wavefront-Nx432", "workgroup-Nx452"])
+
+
+@FP_GRIDS
+def test_fixed_point_matches_oracle(dev, diamond400, g, half_x):
     """The round-trip fixed point with a threading line, on a small grid:
     the same sweep count as the NumPy restatement and the same exit wave;
     a batch of angles equals one angle at a time."""
-    cfg = BraggConfig(diamond400, t_crystal=4e-6, dx=0.1e-6, half_x=2e-6)
+    cfg = BraggConfig(diamond400, t_crystal=4e-6, dx=0.1e-6, half_x=half_x)
     seg = threading_segment("Edge", +1, diamond400.b_mag, float(g[2]),
                             x0=0.3e-6, y0=0.05e-6)
     s = deviation_from_urad(np.array([-6.5, 2.0, 15.0]), diamond400)
@@ -278,8 +285,9 @@ def test_fixed_point_matches_oracle(dev, diamond400, g):
         assert np.array_equal(one, ews[j]) and oi == infos[j]
 
 
-def test_fixed_point_perfect_and_unconverged(dev, diamond400, g):
-    cfg = BraggConfig(diamond400, t_crystal=4e-6, dx=0.1e-6, half_x=3e-6)
+@FP_GRIDS
+def test_fixed_point_perfect_and_unconverged(dev, diamond400, g, half_x):
+    cfg = BraggConfig(diamond400, t_crystal=4e-6, dx=0.1e-6, half_x=half_x)
     sd = float(deviation_from_urad(-6.5, diamond400))
     ew, info = BS.solve_bragg([], sd, cfg, diamond400, g, n_iter=800,
                               tol=1e-13)
