@@ -45,7 +45,7 @@ def parse():
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the batched-ensembles leg")
     ap.add_argument("--probe-bragg", action="store_true",
-                    help="run only cfg1 Case B batches (independent planes, "
+                    help="run only the 20 um Bragg surface batch with one edge line "
                          "one warm-up and five timed), for a kernel trace, "
                          "and exit")
     ap.add_argument("--probe", action="store_true",
@@ -88,14 +88,15 @@ def time_laue(N, segs, s, grid, xtal, prec, steps, warmup):
     return wall, phase_ms, march_ms, out
 
 
-def time_bragg(xtal, case, repeats=3, planes="independent"):
-    """cfg1: 41 s_dev over one 32-plane chunk, solved as one batched
+def time_bragg(xtal, crystal, repeats=3, planes="independent"):
+    """20 um Bragg surface: 41 s_dev over one 32-plane chunk (Nz 90, Nx
+    520), solved as one batched
     GMRES per repeat (cold start); returns solves/s and per-solve info.
     planes="joint" is the reference's chunk (one Krylov space for the 32
     planes of an angle), "independent" one Krylov space per plane."""
     from dyfox_amd.bragg import solve_surface_batch
     from dyfox_amd.workloads import bragg_surface
-    cfg, segs, ys, s, g = bragg_surface(xtal, case)
+    cfg, segs, ys, s, g = bragg_surface(xtal, crystal)
     solve_surface_batch(segs, s, cfg, ys, xtal, g, planes=planes)  # warm-up
     times = []
     for _ in range(repeats):
@@ -113,7 +114,7 @@ def time_bragg(xtal, case, repeats=3, planes="independent"):
 
 
 def time_ensembles():
-    """cfg4: 256 uneven Laue instances (two reflections, t 50-300 um,
+    """256 uneven Laue instances (two reflections, t 50-300 um,
     1-40 segments, 16-128 angles) through the persistent batched path:
     autotuned launch shape, graph captured on the first run, the second
     (replayed) run timed, FP64 then mixed."""
@@ -142,7 +143,7 @@ def time_ensembles():
                           "setup_s": setup}
         sv.close()
         del sv
-        print(f"[bench] cfg4 {prec.name} {ps / dt:.3e} pixel-steps/s "
+        print(f"[bench] ensemble-256 {prec.name} {ps / dt:.3e} pixel-steps/s "
               f"({dt:.2f} s)", file=sys.stderr, flush=True)
     out["n_instances"] = len(inst)
     return out
@@ -239,13 +240,13 @@ def warm_start_report():
     """Cold vs learned-warm-start Bragg solves/s from the committed run of
     `python -m dyfox_amd.warmstart.experiment --train 20000 --test 2000`
     (about six minutes of GPU time, so it is not repeated here)."""
-    path = os.path.join(ROOT, "profiles", "bragg_warmstart_cfg5.json")
+    path = os.path.join(ROOT, "profiles", "bragg_warm_start_20k.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         r = json.load(f)
     v = r["variants"]
-    return {"source": "profiles/bragg_warmstart_cfg5.json (committed run)",
+    return {"source": "profiles/bragg_warm_start_20k.json (committed run)",
             "n_test": r["n"], "n_train": r["n_train"],
             "cold_solves_per_s": v["cold"]["solves_per_s"],
             "warm_solves_per_s": v["network"]["solves_per_s"],
@@ -260,7 +261,7 @@ def warm_start_report():
 
 def traffic_from_profile():
     """HBM bytes per march launch from the committed PMC summary."""
-    path = os.path.join(ROOT, "profiles", "laue_march_cfg3_traffic.json")
+    path = os.path.join(ROOT, "profiles", "laue_march_rocking_scan_273um_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -297,13 +298,14 @@ def probe():
 
 
 def probe_bragg():
-    """cfg1 Case B, independent planes: one warm-up batch, five timed."""
+    """The 20 um Bragg surface batch with one edge line, independent
+    planes: one warm-up batch, five timed."""
     from dyfox_amd.backend import device
     from dyfox_amd.crystal import DIAMOND, CrystalParams
     device()
     xtal = CrystalParams.from_reflection(DIAMOND, (4, 0, 0), 17.0,
                                          f_prime=0.003, f_dprime=0.002)
-    r = time_bragg(xtal, "B", repeats=5)
+    r = time_bragg(xtal, "edge_line", repeats=5)
     print(f"probe-bragg done: {r['ms_per_batch']:.2f} ms per batch, "
           f"iterations {r['gmres_iterations']}", flush=True)
 
@@ -331,7 +333,7 @@ def main():
     grid, segs, s = laue_rocking_scan(xtal)
     work = grid.pixel_steps(len(s))
     steps, warm = max(1, args.steps), max(0, args.warmup)
-    print(f"[bench] cfg3 {grid} n_s={len(s)} n_seg={len(segs)} "
+    print(f"[bench] rocking scan {grid} n_s={len(s)} n_seg={len(segs)} "
           f"work={work:.3e} pixel-steps", file=sys.stderr, flush=True)
 
     w64, p64, m64, out64 = time_laue(N, segs, s, grid, xtal, FP64, steps,
@@ -353,13 +355,15 @@ def main():
     w2, p2, m2, _ = time_laue(N, seg2, s2, g2, xtal, FP64, max(3, steps // 2),
                               1)
     work2 = g2.pixel_steps(len(s2))
-    print(f"[bench] cfg2 fp64 step {w2 * 1e3:.2f} ms", file=sys.stderr,
+    print(f"[bench] worked example fp64 step {w2 * 1e3:.2f} ms", file=sys.stderr,
           flush=True)
 
-    bragg = {c: time_bragg(xtal, c) for c in ("A", "B")}
-    bragg["B_joint_chunk"] = time_bragg(xtal, "B", planes="joint")
-    print(f"[bench] bragg cfg1 B {bragg['B']['solves_per_s']:.1f} solves/s "
-          f"(joint chunk {bragg['B_joint_chunk']['solves_per_s']:.1f})",
+    bragg = {c: time_bragg(xtal, c) for c in ("perfect", "edge_line")}
+    bragg["edge_line_joint_chunk"] = time_bragg(xtal, "edge_line",
+                                                planes="joint")
+    print(f"[bench] bragg surface, edge line: "
+          f"{bragg['edge_line']['solves_per_s']:.1f} solves/s (joint chunk "
+          f"{bragg['edge_line_joint_chunk']['solves_per_s']:.1f})",
           file=sys.stderr, flush=True)
 
     achieved = FLOPS_PER_PIXEL_STEP * work / (m64 * 1e-3) / 1e12
@@ -378,7 +382,7 @@ def main():
         "ms_per_step": w64 * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded 8-segment <110> dislocation field)",
-        "config": {"workload": "laue_rocking_scan_cfg3",
+        "config": {"workload": "laue_rocking_scan_273um_64angles",
                    "crystal": "diamond(400) 17 keV", "t_um": 273,
                    "dx_um": 0.25, "Ny": 256, "Nx": 1056, "Nz": grid.Nz,
                    "n_s_dev": len(s), "n_segments": len(segs),
@@ -395,11 +399,11 @@ def main():
         "worked_example_fp64": {"value": work2 / w2, "ms_per_step": w2 * 1e3,
                                 "phase_ms": p2, "march_ms": m2,
                                 "pixel_steps": work2},
-        "bragg_cfg1_cold": bragg,
+        "bragg_surface_20um_cold": bragg,
         "bragg_fixed_point": time_fixed_point(N, xtal),
-        "imaging_cfg3": imaging,
-        "ensemble_cfg4": None if args.no_ensemble else time_ensembles(),
-        "bragg_warm_start_cfg5": warm_start_report(),
+        "imaging_rocking_scan_273um": imaging,
+        "laue_ensemble_256": None if args.no_ensemble else time_ensembles(),
+        "bragg_warm_start_20k": warm_start_report(),
         "roofline": roof,
     }
     if not args.no_cpu:

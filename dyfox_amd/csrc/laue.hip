@@ -329,7 +329,7 @@ __device__ __forceinline__ void fast_row(const FastArgs &a, int r, int ie) {
 // started and the allocator needs no copies for the loop-carried state
 // (C = 17 FP64: 289 VALU instructions per step instead of 339; mixed 489
 // instead of 587).  In the persistent kernel (capped at two waves per
-// SIMD) this measured faster (cfg4 mixed 4.80 s -> 4.10 s); in the
+// SIMD) this measured faster (256-instance ensemble, mixed: 4.80 s -> 4.10 s); in the
 // stand-alone kernel the same source measured 5-7 % slower than the
 // rolled fast_row above, so the two are kept apart.
 // The persistent kernel reads its table, coefficient and output pointers

@@ -8,10 +8,11 @@
                                 256 rows (Nz 490, Nx 1056), 8 segments
                                 from the <110> family, 64 s_dev in
                                 [-200, +200] urad
-    bragg_surface(xtal, case)   t = 20 um, dx = 0.5 um, half_x = 32 um
+    bragg_surface(xtal, crystal)  t = 20 um, dx = 0.5 um, half_x = 32 um
                                 (Nz 90, Nx 520), 32 planes, 41 s_dev in
-                                [-60, 60] urad; case "A" perfect, "B" one
-                                [101] edge line at mid-depth
+                                [-60, 60] urad; crystal "perfect", or
+                                "edge_line": one [101] edge line at
+                                mid-depth
 
 All draws use numpy.random.default_rng(20260909) unless a generator is
 passed.  Cores are kept half a pixel off the sampled rows.
@@ -113,13 +114,16 @@ def ensemble_batch(n=256, rng=None, dx=0.25e-6):
     return out
 
 
-def bragg_surface(xtal, case="B", n_planes=32, n_s=41):
+def bragg_surface(xtal, crystal="edge_line", n_planes=32, n_s=41):
     from .bragg.geometry import BraggConfig, bragg_g_vector
     cfg = BraggConfig(xtal, t_crystal=20e-6, dx=0.5e-6, half_x=32e-6)
     ys = (np.arange(n_planes) - n_planes // 2) * cfg.dx + cfg.dx / 2
     s = deviation_from_urad(np.linspace(-60, 60, n_s), xtal)
+    if crystal not in ("perfect", "edge_line"):
+        raise ValueError(f"crystal is 'perfect' or 'edge_line', not "
+                         f"{crystal!r}")
     segs = []
-    if case == "B":
+    if crystal == "edge_line":
         xi = np.array([1.0, 0.0, 1.0]) / np.sqrt(2)
         b = xtal.b_mag * np.array([1.0, 0.0, -1.0]) / np.sqrt(2)
         core = np.array([0.0, cfg.dx / 2, cfg.t_crystal / 2])

@@ -241,11 +241,11 @@ def test_warm_start_reaches_the_same_answer_in_fewer_iterations(dev,
     assert _rel(warm, cold) < 1e-8
 
 
-def test_cfg1_case_b_matches_oracle(dev, diamond400, g):
-    """The full cfg1 chunk (Nz 90, Nx 520, 32 planes) with one [101] edge
+def test_20um_surface_with_edge_line_matches_oracle(dev, diamond400, g):
+    """The full 20 um surface chunk (Nz 90, Nx 520, 32 planes) with one [101] edge
     line at mid-depth, two angles checked against the oracle."""
     from dyfox_amd.workloads import bragg_surface
-    cfg, segs, ys, s, gv = bragg_surface(diamond400, "B")
+    cfg, segs, ys, s, gv = bragg_surface(diamond400, "edge_line")
     pick = s[[3, 20]]
     out, infos = solve_surface_batch(segs, pick, cfg, ys, diamond400, gv)
     for j, sd in enumerate(pick):

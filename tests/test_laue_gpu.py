@@ -237,7 +237,7 @@ def test_odd_column_counts(dev, diamond400):
         _check(got, oracle_exit(seg, s, grid, diamond400), 1e-10, str(grid))
 
 
-def test_full_size_cfg2_rows_match_oracle(dev, diamond400):
+def test_full_size_worked_example_rows_match_oracle(dev, diamond400):
     """The worked-example grid (Nz 359, 544 x 800, 481 marched rows,
     s_dev in {0, 117.5 urad}): every row is independent, so the oracle
     checks a spread of rows at full size."""
@@ -255,12 +255,12 @@ def test_full_size_cfg2_rows_match_oracle(dev, diamond400):
     H = OV.h_grid(seg, diamond400.g_vec, diamond400.nu, diamond400.a_core,
                   g["Nx"], g["Ny"], grid.Nz, g["dx"], g["dz"], rows=rows)
     want = OL.march_grid(g, s, H=H, rows=rows)
-    _check(got[:, rows], want[:, rows], 1e-10, "cfg2-rows")
+    _check(got[:, rows], want[:, rows], 1e-10, "worked-example-rows")
     outside = np.setdiff1d(np.arange(g["Ny"]), grid.marched_rows())
     assert np.abs(got[:, outside]).max() == 0.0
     mixed = to_numpy(solve_dislocation(seg, s, grid, diamond400,
                                        precision=MIXED))
-    _check(mixed, got, 1e-6, "cfg2-mixed")
+    _check(mixed, got, 1e-6, "worked-example-mixed")
 
 
 def test_wave_cache_round_trip(dev, diamond400, small_grid, tmp_path):
