@@ -317,9 +317,9 @@ def main():
     if args.probe_bragg:
         return probe_bragg()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(rows x s_dev shard with no communication; see DESIGN.md)",
-              file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path here "
+              "(the work would shard over rows x s_dev with no "
+              "communication between GPUs)", file=sys.stderr)
         sys.exit(2)
     from dyfox_amd import _native as N
     from dyfox_amd.backend import FP64, MIXED, device

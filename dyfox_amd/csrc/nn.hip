@@ -10,7 +10,7 @@
 //     data      dXcol[p][k] = sum_co dZ[p][co] W[co][k]
 //     weights   dW[co][k]  = sum_p dZ[p][co] Xcol[p][k]   (split over p)
 // followed by col2im for the data gradient.  No counterpart in the
-// reference (north-star addition).
+// reference.
 #include <hip/hip_runtime.h>
 
 #include "common.hpp"

@@ -6,6 +6,6 @@
     model, losses = data.train(ds, n_train)
     report = data.evaluate(xtal, instances, ds, model, held_out)
 
-The plain solver is both the fallback and the judge: every warm solve
+The plain solver is both the fallback and the arbiter: every warm solve
 still runs GMRES to the same true residual.
 """

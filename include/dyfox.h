@@ -152,7 +152,7 @@ int dyf_laue_march(const dyf_laue_args *a, int mode, int prec,
                    const dyf_line_params *line, const double *s_dev_dev,
                    int n_e, void *coef_dev, void *out_dev, void *stream);
 
-/* ---- Batched Laue solves (north-star: persistent launch) ------------- */
+/* ---- Batched Laue solves (persistent launch) -------------------------- */
 /* One instance: its grid and crystal, segments (host), angles and output.
  * Batches of these run as one persistent march (no reference
  * counterpart: the reference launches one grid per solve,
@@ -342,7 +342,7 @@ int dyf_optics_com_accumulate(const void *images_dev, int n, int prec,
                               double *image_sums_dev, void *stream);
 
 /* ---- Learned warm start: BF16-on-MFMA network kernels --------------- */
-/* No counterpart in the reference (north-star addition).  Tensors are
+/* No counterpart in the reference.  Tensors are
  * NHWC; bf16 is stored as uint16 (bfloat16 bits).  A layer reads a channel
  * slice [off, off+C) of a buffer with Ctot channels.                     */
 /* C(i,j) = sum_k A(i,k) B(j,k), A(i,k) = A[i a_i + k a_k] (bf16),

@@ -1,7 +1,8 @@
 """CPU FP64 oracle for the Takagi-Taupin hot path.  TEST INFRASTRUCTURE ONLY.
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
-import this package, and only as the checker (or the timed CPU baseline):
+import this package, and only as the reference results are compared
+against (or the timed CPU baseline):
 the product (dyfox_amd) never calls it, and has no CPU fallback.
 
     volterra.py      H = 2 pi g.u of straight dislocations (NumPy), after
