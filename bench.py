@@ -149,6 +149,31 @@ def time_ensembles():
     return out
 
 
+def time_illumination(N, segs, grid, xtal, repeats=2):
+    """An incoherent image of the rocking-scan field under a convergent,
+    polychromatic beam: 7 angles x 5 energies (sigma_psi 20 urad,
+    sigma_delta 1e-4), each node marched as one angle, imaged through its
+    own chromatic charge-2 pupil, and summed with its weight in FP64 on
+    the device (illumination.incoherent_image).  Timed end to end,
+    including the one e^{iH} table of the field."""
+    from dyfox_amd.illumination import (convergence_bandwidth_nodes,
+                                        incoherent_image)
+    from dyfox_amd.optics import Optics
+    lens = Optics(1e-4, 0.274, xtal)
+    s, d, w, info = convergence_bandwidth_nodes(xtal, 20e-6, 1e-4, 7, 60e-6,
+                                                5, 3e-4)
+    incoherent_image(segs, grid, xtal, lens, s, w, d_arg=d, ell=2)
+    N.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(repeats):
+        img = incoherent_image(segs, grid, xtal, lens, s, w, d_arg=d, ell=2)
+    N.synchronize()
+    dt = (time.perf_counter() - t0) / repeats
+    del img
+    return {"nodes_per_s": len(s) / dt, "ms_per_image": dt * 1e3,
+            "nodes": len(s), "grid": "256 x 1056, Nz 490"}
+
+
 def time_imaging(N, stack, grid, xtal, repeats=5):
     """The exit-wave stack of the rocking scan (64 x 256 x 1056 complex)
     imaged through the open and the charge-2 pupil (Optics.image: one
@@ -350,6 +375,9 @@ def main():
     print(f"[bench] imaging {imaging['images_per_s']:.0f} images/s",
           file=sys.stderr, flush=True)
     del out64, outmx
+    illum = time_illumination(N, segs, grid, xtal)
+    print(f"[bench] incoherent image, {illum['nodes']} nodes: "
+          f"{illum['ms_per_image']:.1f} ms", file=sys.stderr, flush=True)
 
     g2, seg2, s2 = laue_worked_example(xtal)
     w2, p2, m2, _ = time_laue(N, seg2, s2, g2, xtal, FP64, max(3, steps // 2),
@@ -402,6 +430,7 @@ def main():
         "bragg_surface_20um_cold": bragg,
         "bragg_fixed_point": time_fixed_point(N, xtal),
         "imaging_rocking_scan_273um": imaging,
+        "illumination_rocking_scan_273um": illum,
         "laue_ensemble_256": None if args.no_ensemble else time_ensembles(),
         "bragg_warm_start_20k": warm_start_report(),
         "roofline": roof,

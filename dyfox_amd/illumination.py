@@ -24,7 +24,7 @@ import ctypes as C
 import numpy as np
 
 from . import _native as N
-from .backend import FP64
+from .backend import FP64, precision as _precision
 
 
 def condenser_angles(w_D, reach=75.0):
@@ -94,7 +94,7 @@ def incoherent_image(seg_list, grid, xtal, optics, s_dev, weights,
     at deviation s_dev[n] through the charge-`ell` pupil, walked by the
     chromatic argument d_arg[n] when given (Optics.chromatic).  Returns
     the (Ny, Nx) float64 map as a DeviceArray."""
-    from .laue.solver import solve_dislocation
+    from .laue.solver import march, prepare_phase
     from .optics import image_from_spectrum, object_spectrum
     s_dev = np.asarray(s_dev, np.float64)
     w = np.asarray(weights, np.float64)
@@ -107,10 +107,15 @@ def incoherent_image(seg_list, grid, xtal, optics, s_dev, weights,
     Nx, Ny = grid.geometry()[:2]
     total = N.DeviceArray.zeros((Ny, Nx), np.float64)
     plain = N.DeviceArray.zeros((Ny, Nx), np.float64)
+    # one e^{iH} table for every node (the field does not change), then
+    # each batch of nodes is one march
+    prec = _precision(precision)
+    src = prepare_phase(seg_list, grid, xtal, precision=prec, reuse=True)
+    sd_all = N.DeviceArray.from_host(s_dev)
     for b0 in range(0, len(s_dev), batch):
         b1 = min(b0 + batch, len(s_dev))
-        stack = solve_dislocation(seg_list, s_dev[b0:b1], grid, xtal,
-                                  precision=precision)
+        stack = N.DeviceArray.empty((b1 - b0, Ny, Nx), prec.complex)
+        march(src, sd_all[b0:b1], grid, xtal, stack, prec)
         if d_arg is None:
             imgs = optics.image(stack, grid.dx, ell=ell)
         else:

@@ -16,7 +16,7 @@ import ctypes as C
 import numpy as np
 
 from . import _native as N
-from .backend import FP64
+from .backend import FP64, precision as _precision
 from .units import urad_from_deviation
 
 
@@ -50,17 +50,21 @@ def com_maps(seg_list, s_dev_arr, grid, xtal, optics, ells=(0, 2), *,
     in microradians, accumulated in FP64 across batched solves.  Returns
     (com, rock, weight): the maps, the integrated rocking curve per
     charge, and the summed intensity that weighted them (host arrays)."""
-    from .laue.solver import solve_dislocation
+    from .laue.solver import march, prepare_phase
     s_dev_arr = np.asarray(s_dev_arr, dtype=float)
     Nx, Ny = grid.geometry()[:2]
     phi_urad = np.asarray(urad_from_deviation(s_dev_arr, xtal), np.float64)
     sums_wI = {ell: N.DeviceArray.zeros((Ny, Nx), np.float64) for ell in ells}
     sums_I = {ell: N.DeviceArray.zeros((Ny, Nx), np.float64) for ell in ells}
     rock = {ell: np.zeros(len(s_dev_arr)) for ell in ells}
+    # one e^{iH} table for the whole scan, then one march per batch
+    prec = _precision(precision)
+    src = prepare_phase(seg_list, grid, xtal, precision=prec, reuse=True)
+    sd_all = N.DeviceArray.from_host(s_dev_arr)
     for bs in range(0, len(s_dev_arr), batch):
         be = min(bs + batch, len(s_dev_arr))
-        stack = solve_dislocation(seg_list, s_dev_arr[bs:be], grid, xtal,
-                                  precision=precision)
+        stack = N.DeviceArray.empty((be - bs, Ny, Nx), prec.complex)
+        march(src, sd_all[bs:be], grid, xtal, stack, prec)
         w = N.DeviceArray.from_host(phi_urad[bs:be])
         totals = N.DeviceArray.empty((be - bs,), np.float64)
         for ell in ells:
