@@ -36,6 +36,15 @@ struct CoefF {
     float cr, ci, br, bi;
 };
 
+// The per-step coefficients are uniform per wave and read-only: read
+// through the constant address space they become scalar loads.  As
+// vector loads (the compiler cannot rule out that they alias the output)
+// they were issued after the prefetched e^{iH} slice, so waiting for them
+// waited for the whole slice too (the vector-memory counter drains in
+// issue order).
+template <typename T>
+using kptr = const T __attribute__((address_space(4))) *;
+
 // Per-call workspace: [CoefD n_e*nsteps][CoefF n_e*nsteps][qN 2*n_e doubles]
 static size_t coef_bytes(int n_e, int nsteps) {
     const size_t a = (size_t)n_e * nsteps * sizeof(CoefD);
@@ -212,26 +221,31 @@ __device__ __forceinline__ void fast_row(const FastArgs &a, int r, int ie) {
     if constexpr (MODE == DYF_MODE_ANALYTIC)
         Erow = (const cx<T> *)a.phase + (size_t)r * a.nsteps;
 
-    cx<T> Ec[MODE == DYF_MODE_GRID ? C : 1], En[MODE == DYF_MODE_GRID ? C : 1];
+    // One register set for the e^{iH} slice, prefetched on a rolling
+    // basis: column i's value for step k+1 is loaded into Ec[i] as soon
+    // as column i of step k has used it, so every load has a whole step
+    // to land and no second slice of registers is held (C = 17 FP64: 68
+    // VGPRs fewer, a third wave per SIMD).
+    cx<T> Ec[MODE == DYF_MODE_GRID ? C : 1];
     if constexpr (MODE == DYF_MODE_GRID) {
 #pragma unroll
         for (int i = 0; i < C; ++i) Ec[i] = E[i * kWave];
     }
 
     for (int k = 0; k < a.nsteps; ++k) {
+        const cx<T> *Ek = nullptr;
         if constexpr (MODE == DYF_MODE_GRID) {
             const int kn = k + 1 < a.nsteps ? k + 1 : k;
-            const cx<T> *Ek = E + (size_t)kn * (C * kWave);
-#pragma unroll
-            for (int i = 0; i < C; ++i) En[i] = Ek[i * kWave];
+            Ek = E + (size_t)kn * (C * kWave);
         }
         T cr, ci, br, bi;
+        const size_t ci_ = (size_t)__builtin_amdgcn_readfirstlane(ie) * a.nsteps + k;
         if constexpr (sizeof(T) == 8) {
-            const CoefD c = a.cd[(size_t)ie * a.nsteps + k];
-            cr = c.cr, ci = c.ci, br = c.br, bi = c.bi;
+            kptr<CoefD> c = (kptr<CoefD>)a.cd + ci_;
+            cr = c->cr, ci = c->ci, br = c->br, bi = c->bi;
         } else {
-            const CoefF c = a.cf[(size_t)ie * a.nsteps + k];
-            cr = c.cr, ci = c.ci, br = c.br, bi = c.bi;
+            kptr<CoefF> c = (kptr<CoefF>)a.cf + ci_;
+            cr = c->cr, ci = c->ci, br = c->br, bi = c->bi;
         }
         if constexpr (MODE == DYF_MODE_ANALYTIC) {
             // fold the row-uniform e^{iH_k} into the coefficients
@@ -264,6 +278,10 @@ __device__ __forceinline__ void fast_row(const FastArgs &a, int r, int ie) {
                 ti = fmaT(er, yi, ei * yr);
                 ur = fmaT(er, xr, ei * xi);
                 ui = fmaT(er, xi, -ei * xr);
+                // the next step's value, into the register just read (the
+                // fence keeps the load from being hoisted into a copy)
+                asm volatile("" ::: "memory");
+                Ec[i] = Ek[i * kWave];
             } else {
                 tr = msk[i] * yr, ti = msk[i] * yi;
                 ur = msk[i] * xr, ui = msk[i] * xi;
@@ -299,10 +317,6 @@ __device__ __forceinline__ void fast_row(const FastArgs &a, int r, int ie) {
             if constexpr (COMP)
                 Xlr[i] = nXlr[i], Xli[i] = nXli[i], Ylr[i] = nYlr[i],
                 Yli[i] = nYli[i];
-        }
-        if constexpr (MODE == DYF_MODE_GRID) {
-#pragma unroll
-            for (int i = 0; i < C; ++i) Ec[i] = En[i];
         }
     }
 
