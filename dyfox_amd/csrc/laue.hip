@@ -377,8 +377,6 @@ __device__ __forceinline__ void fast_row_unrolled(const FastArgs &a, int r,
         if constexpr (COMP) Xlr[i] = Xli[i] = Ylr[i] = Yli[i] = T(0);
     }
     gptr<cx<T>> E = (gptr<cx<T>>)a.phase + (size_t)r * a.nsteps * kSlice + lane;
-    gptr<CoefD> cdg = (gptr<CoefD>)a.cd;
-    gptr<CoefF> cfg = (gptr<CoefF>)a.cf;
 
     auto step = [&](int k) {
         cx<T> Ec[C];
@@ -387,11 +385,12 @@ __device__ __forceinline__ void fast_row_unrolled(const FastArgs &a, int r,
         for (int i = 0; i < C; ++i)
             Ec[i] = cx<T>{Ek[i * kWave].r, Ek[i * kWave].i};
         T cr, ci, br, bi;
+        const size_t ci_ = (size_t)__builtin_amdgcn_readfirstlane(ie) * a.nsteps + k;
         if constexpr (sizeof(T) == 8) {
-            gptr<CoefD> c = cdg + (size_t)ie * a.nsteps + k;
+            kptr<CoefD> c = (kptr<CoefD>)a.cd + ci_;
             cr = c->cr, ci = c->ci, br = c->br, bi = c->bi;
         } else {
-            gptr<CoefF> c = cfg + (size_t)ie * a.nsteps + k;
+            kptr<CoefF> c = (kptr<CoefF>)a.cf + ci_;
             cr = c->cr, ci = c->ci, br = c->br, bi = c->bi;
         }
         // inflow across lane boundaries (old values)
