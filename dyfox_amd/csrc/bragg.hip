@@ -449,7 +449,8 @@ constexpr int kRedThreads = 256;
 __global__ __launch_bounds__(kRedThreads) void batched_dot_kernel(
     const cd *__restrict__ A, size_t a_sys, size_t a_vec, int kc,
     const cd *__restrict__ Bv, size_t b_sys, size_t N, int P,
-    cd *__restrict__ part, const int *__restrict__ active) {
+    cd *__restrict__ part, const int *__restrict__ active,
+    cd *__restrict__ out, int ostride) {
     __shared__ double rr[kRedThreads], ri[kRedThreads];
     const int sq = blockIdx.x, p = blockIdx.y;
     const int s = sq / kc, q = sq % kc;
@@ -474,7 +475,13 @@ __global__ __launch_bounds__(kRedThreads) void batched_dot_kernel(
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) part[(size_t)sq * P + p] = cd{rr[0], ri[0]};
+    // one part (N <= 4096): the block's sum is the dot product itself
+    if (threadIdx.x == 0) {
+        if (P == 1)
+            out[(size_t)s * ostride + q] = cd{rr[0], ri[0]};
+        else
+            part[(size_t)sq * P + p] = cd{rr[0], ri[0]};
+    }
 }
 
 // out[s*ostride + q] = sum_p part[...] (fixed order: deterministic)
@@ -765,8 +772,9 @@ static int dots(const cd *A, size_t a_sys, size_t a_vec, int kc, const cd *Bv,
                 int nsys, const int *active, hipStream_t stream) {
     const int P = reduce_parts(N);
     batched_dot_kernel<<<dim3(nsys * kc, P), kRedThreads, 0, stream>>>(
-        A, a_sys, a_vec, kc, Bv, b_sys, N, P, part, active);
+        A, a_sys, a_vec, kc, Bv, b_sys, N, P, part, active, out, ostride);
     DYF_LAUNCHED("batched_dot_kernel");
+    if (P == 1) return DYF_OK;            // written to out directly
     const int n = nsys * kc;
     sum_parts_kernel<<<(n + 255) / 256, 256, 0, stream>>>(part, P, n, out,
                                                           ostride, active, kc);
