@@ -647,6 +647,106 @@ __global__ void arnoldi_finish_kernel(int nsys, int restart, int col, cd *H,
 
 __global__ void zero_int_kernel(int *p) { *p = 0; }
 
+// CGS2 of one Arnoldi step for systems of N <= kCgsMaxN unknowns, one
+// workgroup per system with w in registers: nrm0 = ||w||^2, then twice
+// h = V^H w (the kc = col + 1 basis vectors), w -= V h, then nrm = ||w||^2
+// (what the dot/combine kernels do in six launches).  Reductions: a
+// butterfly within each wavefront, then the four wavefront sums in a
+// fixed order.
+constexpr int kCgsThreads = 256;
+constexpr int kCgsPer = 8;
+constexpr int kCgsMaxN = kCgsThreads * kCgsPer;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__global__ __launch_bounds__(kCgsThreads) void cgs2_kernel(
+    const cd *__restrict__ V, size_t v_sys, size_t N, int kc,
+    cd *__restrict__ w, cd *__restrict__ nrm0, cd *__restrict__ h1,
+    cd *__restrict__ h2, int hstride, cd *__restrict__ nrm,
+    const int *__restrict__ active) {
+    extern __shared__ double red[];           // [4 waves][2 kc]
+    const int s = blockIdx.x, tid = threadIdx.x;
+    if (!active[s]) return;
+    const int wv = tid >> 6, lane = tid & 63;
+    cd *ws = w + (size_t)s * N;
+    const cd *Vs = V + (size_t)s * v_sys;
+    cd x[kCgsPer];
+    double nn = 0.0;
+#pragma unroll
+    for (int m = 0; m < kCgsPer; ++m) {
+        const size_t i = tid + (size_t)m * kCgsThreads;
+        x[m] = i < N ? ws[i] : cd{0.0, 0.0};
+        nn += x[m].r * x[m].r + x[m].i * x[m].i;
+    }
+    // block sum of one value through red[0..3]
+    auto block_sum = [&](double v) {
+        v = wave_sum(v);
+        __syncthreads();
+        if (lane == 0) red[wv] = v;
+        __syncthreads();
+        return ((red[0] + red[1]) + red[2]) + red[3];
+    };
+    nn = block_sum(nn);
+    if (tid == 0) nrm0[s] = cd{nn, 0.0};
+    for (int pass = 0; pass < 2; ++pass) {
+        cd *hp = (pass == 0 ? h1 : h2) + (size_t)s * hstride;
+        __syncthreads();
+        for (int q = 0; q < kc; ++q) {
+            const cd *v = Vs + (size_t)q * N;
+            double sr = 0.0, si = 0.0;
+#pragma unroll
+            for (int m = 0; m < kCgsPer; ++m) {
+                const size_t i = tid + (size_t)m * kCgsThreads;
+                if (i < N) {
+                    const cd a = v[i];
+                    sr += a.r * x[m].r + a.i * x[m].i;
+                    si += a.r * x[m].i - a.i * x[m].r;
+                }
+            }
+            sr = wave_sum(sr), si = wave_sum(si);
+            if (lane == 0) red[wv * 2 * kc + 2 * q] = sr,
+                           red[wv * 2 * kc + 2 * q + 1] = si;
+        }
+        __syncthreads();
+        if (tid < kc) {
+            const double r = ((red[2 * tid] + red[2 * kc + 2 * tid]) +
+                              red[4 * kc + 2 * tid]) + red[6 * kc + 2 * tid];
+            const double i = ((red[2 * tid + 1] + red[2 * kc + 2 * tid + 1]) +
+                              red[4 * kc + 2 * tid + 1]) +
+                             red[6 * kc + 2 * tid + 1];
+            hp[tid] = cd{r, i};
+            red[8 * kc + 2 * tid] = r, red[8 * kc + 2 * tid + 1] = i;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < kCgsPer; ++m) {
+            const size_t i = tid + (size_t)m * kCgsThreads;
+            if (i >= N) continue;
+            double ar = 0.0, ai = 0.0;
+            for (int q = 0; q < kc; ++q) {
+                const double cr = red[8 * kc + 2 * q], ci = red[8 * kc + 2 * q + 1];
+                const cd b = Vs[(size_t)q * N + i];
+                ar += cr * b.r - ci * b.i;
+                ai += cr * b.i + ci * b.r;
+            }
+            x[m].r -= ar, x[m].i -= ai;
+        }
+    }
+    nn = 0.0;
+#pragma unroll
+    for (int m = 0; m < kCgsPer; ++m) {
+        const size_t i = tid + (size_t)m * kCgsThreads;
+        if (i < N) ws[i] = x[m];
+        nn += x[m].r * x[m].r + x[m].i * x[m].i;
+    }
+    nn = block_sum(nn);
+    if (tid == 0) nrm[s] = cd{nn, 0.0};
+}
+
 // Start of a restart cycle: inv_s = 1/||M r_s||, S_s = [||M r_s||, 0, ...]
 // (the first column of the Arnoldi basis is M r_s * inv_s).
 __global__ void cycle_start_kernel(int nsys, int R, const cd *nrm,
@@ -1361,22 +1461,30 @@ extern "C" int dyf_bragg_solve(const dyf_bragg_args *a, const double *s_dev,
             rc = apply_precon(k, W.pinv, W.av, W.w, stream);
             if (rc) return rc;
             // h0 = ||w||; CGS2 against v_0..v_col
-            rc = dots(W.w, N, 0, 1, W.w, N, N, W.nrm0, 1, W.part, nsys,
-                      W.active, stream);
-            if (rc) return rc;
-            for (int pass = 0; pass < 2; ++pass) {
-                cd *hp = pass == 0 ? W.h1 : W.h2;
-                rc = dots(W.V, Vsys, N, col + 1, W.w, N, N, hp, R + 1, W.part,
-                          nsys, W.active, stream);
+            if (N <= (size_t)kCgsMaxN) {
+                const size_t lds = sizeof(double) * 10 * (size_t)(col + 1);
+                cgs2_kernel<<<nsys, kCgsThreads, std::max(lds, sizeof(double) * 4),
+                              stream>>>(W.V, Vsys, N, col + 1, W.w, W.nrm0,
+                                        W.h1, W.h2, R + 1, W.nrm, W.active);
+                DYF_LAUNCHED("cgs2_kernel");
+            } else {
+                rc = dots(W.w, N, 0, 1, W.w, N, N, W.nrm0, 1, W.part, nsys,
+                          W.active, stream);
                 if (rc) return rc;
-                combine_kernel<<<vgrid, 256, 0, stream>>>(
-                    W.V, Vsys, N, col + 1, nullptr, hp, R + 1, -1.0, W.w, N, N,
-                    W.active);
-                DYF_LAUNCHED("combine_kernel");
+                for (int pass = 0; pass < 2; ++pass) {
+                    cd *hp = pass == 0 ? W.h1 : W.h2;
+                    rc = dots(W.V, Vsys, N, col + 1, W.w, N, N, hp, R + 1,
+                              W.part, nsys, W.active, stream);
+                    if (rc) return rc;
+                    combine_kernel<<<vgrid, 256, 0, stream>>>(
+                        W.V, Vsys, N, col + 1, nullptr, hp, R + 1, -1.0, W.w,
+                        N, N, W.active);
+                    DYF_LAUNCHED("combine_kernel");
+                }
+                rc = dots(W.w, N, 0, 1, W.w, N, N, W.nrm, 1, W.part, nsys,
+                          W.active, stream);
+                if (rc) return rc;
             }
-            rc = dots(W.w, N, 0, 1, W.w, N, N, W.nrm, 1, W.part, nsys,
-                      W.active, stream);
-            if (rc) return rc;
             zero_int_kernel<<<1, 1, 0, stream>>>(W.still);
             DYF_LAUNCHED("zero_int_kernel");
             arnoldi_finish_kernel<<<(nsys + 63) / 64, 64, 0, stream>>>(
