@@ -583,18 +583,16 @@ __device__ void lartg(cd f, cd g, double *c, cd *s, cd *r) {
 // the stored Givens rotations and a new one, the residual estimate, and
 // SciPy's inner-loop exit (presid <= ptol, breakdown or the last column),
 // which clears the system's active flag; `still` is set while any system
-// continues, so the host reads one int per column.
-__global__ void arnoldi_finish_kernel(int nsys, int restart, int col, cd *H,
-                                      cd *G, cd *S, const cd *h1st,
-                                      const cd *h2nd, const cd *nrm_w,
-                                      const cd *nrm0, SysState *st,
-                                      double *inv, int *active, double eps,
-                                      int *still) {
-    const int s = blockIdx.x * 64 + threadIdx.x;
-    if (s >= nsys || !active[s]) return;
+// continues, so the host reads one int per column.  nw = ||w||^2 after
+// CGS2, n0 = ||w||^2 before it.  Returns whether system s continues.
+__device__ int arnoldi_finish_sys(int s, int restart, int col, cd *H, cd *G,
+                                  cd *S, const cd *h1st, const cd *h2nd,
+                                  double nw, double n0, SysState *st,
+                                  double *inv, int *active, double eps,
+                                  int *still) {
     SysState &q = st[s];
     q.col = col;
-    q.h0 = sqrt(nrm0[s].r);
+    q.h0 = sqrt(n0);
     cd *h = H + (size_t)s * restart * (restart + 1) + (size_t)col * (restart + 1);
     cd *g = G + (size_t)s * restart * 2;
     cd *Sv = S + (size_t)s * (restart + 1);
@@ -603,7 +601,7 @@ __global__ void arnoldi_finish_kernel(int nsys, int restart, int col, cd *H,
         const cd b = h2nd[(size_t)s * (restart + 1) + r];
         h[r] = cd{a.r + b.r, a.i + b.i};
     }
-    const double h1 = sqrt(nrm_w[s].r);
+    const double h1 = sqrt(nw);
     h[col + 1] = cd{h1, 0.0};
     int breakdown = 0;
     if (h1 <= eps * q.h0) {
@@ -640,19 +638,32 @@ __global__ void arnoldi_finish_kernel(int nsys, int restart, int col, cd *H,
     if (q.presid <= q.ptol || breakdown || col == restart - 1) {
         q.inner_done = 1;
         active[s] = 0;
-    } else {
-        atomicOr(still, 1);
+        return 0;
     }
+    atomicOr(still, 1);
+    return 1;
 }
 
-__global__ void zero_int_kernel(int *p) { *p = 0; }
+__global__ void arnoldi_finish_kernel(int nsys, int restart, int col, cd *H,
+                                      cd *G, cd *S, const cd *h1st,
+                                      const cd *h2nd, const cd *nrm_w,
+                                      const cd *nrm0, SysState *st,
+                                      double *inv, int *active, double eps,
+                                      int *still) {
+    const int s = blockIdx.x * 64 + threadIdx.x;
+    if (s >= nsys || !active[s]) return;
+    arnoldi_finish_sys(s, restart, col, H, G, S, h1st, h2nd, nrm_w[s].r,
+                       nrm0[s].r, st, inv, active, eps, still);
+}
 
-// CGS2 of one Arnoldi step for systems of N <= kCgsMaxN unknowns, one
-// workgroup per system with w in registers: nrm0 = ||w||^2, then twice
-// h = V^H w (the kc = col + 1 basis vectors), w -= V h, then nrm = ||w||^2
-// (what the dot/combine kernels do in six launches).  Reductions: a
-// butterfly within each wavefront, then the four wavefront sums in a
-// fixed order.
+// One Arnoldi step after w = M A v_col for systems of N <= kCgsMaxN
+// unknowns, one workgroup per system with w in registers: CGS2 (||w||^2,
+// then twice h = V^H w over the kc = col + 1 basis vectors and w -= V h,
+// then ||w||^2), the Hessenberg/Givens update of arnoldi_finish_sys on
+// one lane, and v_{col+1} = w / ||w|| for a system that continues (what
+// the dot, combine, finish and scale kernels do in nine launches).
+// Reductions: a butterfly within each wavefront, then the four wavefront
+// sums in a fixed order.
 constexpr int kCgsThreads = 256;
 constexpr int kCgsPer = 8;
 constexpr int kCgsMaxN = kCgsThreads * kCgsPer;
@@ -663,23 +674,34 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-__global__ __launch_bounds__(kCgsThreads) void cgs2_kernel(
-    const cd *__restrict__ V, size_t v_sys, size_t N, int kc,
-    cd *__restrict__ w, cd *__restrict__ nrm0, cd *__restrict__ h1,
-    cd *__restrict__ h2, int hstride, cd *__restrict__ nrm,
-    const int *__restrict__ active) {
-    extern __shared__ double red[];           // [4 waves][2 kc]
+// P = elements of w per lane (N <= 256 P).  Every lane loads at a
+// clamped in-range index, so the loads of a basis vector carry no branch
+// and two vectors' loads are in flight together; lanes past N hold w = 0
+// and keep it.
+template <int P>
+__global__ __launch_bounds__(kCgsThreads) void arnoldi_step_kernel(
+    cd *__restrict__ V, size_t v_sys, size_t N, int restart, int col,
+    const cd *__restrict__ w, cd *__restrict__ h1, cd *__restrict__ h2,
+    cd *H, cd *G, cd *S, SysState *st, double *inv, int *active, double eps,
+    int *still) {
+    extern __shared__ double red[];   // [4 waves][2 kc], h [2 kc]
     const int s = blockIdx.x, tid = threadIdx.x;
     if (!active[s]) return;
     const int wv = tid >> 6, lane = tid & 63;
-    cd *ws = w + (size_t)s * N;
-    const cd *Vs = V + (size_t)s * v_sys;
-    cd x[kCgsPer];
+    const int kc = col + 1, hstride = restart + 1;
+    const cd *ws = w + (size_t)s * N;
+    cd *Vs = V + (size_t)s * v_sys;
+    cd x[P];
+    size_t idx[P];
+    bool ok[P];
     double nn = 0.0;
 #pragma unroll
-    for (int m = 0; m < kCgsPer; ++m) {
+    for (int m = 0; m < P; ++m) {
         const size_t i = tid + (size_t)m * kCgsThreads;
-        x[m] = i < N ? ws[i] : cd{0.0, 0.0};
+        ok[m] = i < N;
+        idx[m] = ok[m] ? i : N - 1;
+        const cd v = ws[idx[m]];
+        x[m] = ok[m] ? v : cd{0.0, 0.0};
         nn += x[m].r * x[m].r + x[m].i * x[m].i;
     }
     // block sum of one value through red[0..3]
@@ -690,26 +712,33 @@ __global__ __launch_bounds__(kCgsThreads) void cgs2_kernel(
         __syncthreads();
         return ((red[0] + red[1]) + red[2]) + red[3];
     };
-    nn = block_sum(nn);
-    if (tid == 0) nrm0[s] = cd{nn, 0.0};
+    const double n0 = block_sum(nn);
     for (int pass = 0; pass < 2; ++pass) {
         cd *hp = (pass == 0 ? h1 : h2) + (size_t)s * hstride;
         __syncthreads();
-        for (int q = 0; q < kc; ++q) {
-            const cd *v = Vs + (size_t)q * N;
-            double sr = 0.0, si = 0.0;
+        for (int q = 0; q < kc; q += 2) {
+            const bool two = q + 1 < kc;
+            const cd *v0 = Vs + (size_t)q * N;
+            const cd *v1 = Vs + (size_t)(two ? q + 1 : q) * N;
+            cd a[P], b[P];
 #pragma unroll
-            for (int m = 0; m < kCgsPer; ++m) {
-                const size_t i = tid + (size_t)m * kCgsThreads;
-                if (i < N) {
-                    const cd a = v[i];
-                    sr += a.r * x[m].r + a.i * x[m].i;
-                    si += a.r * x[m].i - a.i * x[m].r;
-                }
+            for (int m = 0; m < P; ++m) a[m] = v0[idx[m]], b[m] = v1[idx[m]];
+            double sr = 0.0, si = 0.0, tr = 0.0, ti = 0.0;
+#pragma unroll
+            for (int m = 0; m < P; ++m) {
+                sr += a[m].r * x[m].r + a[m].i * x[m].i;
+                si += a[m].r * x[m].i - a[m].i * x[m].r;
+                tr += b[m].r * x[m].r + b[m].i * x[m].i;
+                ti += b[m].r * x[m].i - b[m].i * x[m].r;
             }
             sr = wave_sum(sr), si = wave_sum(si);
-            if (lane == 0) red[wv * 2 * kc + 2 * q] = sr,
-                           red[wv * 2 * kc + 2 * q + 1] = si;
+            tr = wave_sum(tr), ti = wave_sum(ti);
+            if (lane == 0) {
+                red[wv * 2 * kc + 2 * q] = sr, red[wv * 2 * kc + 2 * q + 1] = si;
+                if (two)
+                    red[wv * 2 * kc + 2 * q + 2] = tr,
+                    red[wv * 2 * kc + 2 * q + 3] = ti;
+            }
         }
         __syncthreads();
         if (tid < kc) {
@@ -722,36 +751,71 @@ __global__ __launch_bounds__(kCgsThreads) void cgs2_kernel(
             red[8 * kc + 2 * tid] = r, red[8 * kc + 2 * tid + 1] = i;
         }
         __syncthreads();
+        double ar[P], ai[P];
 #pragma unroll
-        for (int m = 0; m < kCgsPer; ++m) {
-            const size_t i = tid + (size_t)m * kCgsThreads;
-            if (i >= N) continue;
-            double ar = 0.0, ai = 0.0;
-            for (int q = 0; q < kc; ++q) {
-                const double cr = red[8 * kc + 2 * q], ci = red[8 * kc + 2 * q + 1];
-                const cd b = Vs[(size_t)q * N + i];
-                ar += cr * b.r - ci * b.i;
-                ai += cr * b.i + ci * b.r;
+        for (int m = 0; m < P; ++m) ar[m] = 0.0, ai[m] = 0.0;
+        for (int q = 0; q < kc; q += 2) {
+            const bool two = q + 1 < kc;
+            const double c0r = red[8 * kc + 2 * q], c0i = red[8 * kc + 2 * q + 1];
+            const double c1r = two ? red[8 * kc + 2 * q + 2] : 0.0;
+            const double c1i = two ? red[8 * kc + 2 * q + 3] : 0.0;
+            const cd *v0 = Vs + (size_t)q * N;
+            const cd *v1 = Vs + (size_t)(two ? q + 1 : q) * N;
+            cd a[P], b[P];
+#pragma unroll
+            for (int m = 0; m < P; ++m) a[m] = v0[idx[m]], b[m] = v1[idx[m]];
+#pragma unroll
+            for (int m = 0; m < P; ++m) {
+                ar[m] += c0r * a[m].r - c0i * a[m].i;
+                ai[m] += c0r * a[m].i + c0i * a[m].r;
+                ar[m] += c1r * b[m].r - c1i * b[m].i;
+                ai[m] += c1r * b[m].i + c1i * b[m].r;
             }
-            x[m].r -= ar, x[m].i -= ai;
+        }
+#pragma unroll
+        for (int m = 0; m < P; ++m) {
+            x[m].r = ok[m] ? x[m].r - ar[m] : 0.0;
+            x[m].i = ok[m] ? x[m].i - ai[m] : 0.0;
         }
     }
     nn = 0.0;
 #pragma unroll
-    for (int m = 0; m < kCgsPer; ++m) {
-        const size_t i = tid + (size_t)m * kCgsThreads;
-        if (i < N) ws[i] = x[m];
+    for (int m = 0; m < P; ++m)
         nn += x[m].r * x[m].r + x[m].i * x[m].i;
-    }
     nn = block_sum(nn);
-    if (tid == 0) nrm[s] = cd{nn, 0.0};
+    // red[0..3] may still be read by other lanes: broadcast through red[4]
+    if (tid == 0)
+        red[4] = arnoldi_finish_sys(s, restart, col, H, G, S, h1, h2, nn, n0,
+                                    st, inv, active, eps, still)
+                     ? inv[s]
+                     : 0.0;
+    __syncthreads();
+    const double sc = red[4];
+    if (sc == 0.0) return;
+    cd *vn = Vs + (size_t)kc * N;
+#pragma unroll
+    for (int m = 0; m < P; ++m)
+        if (ok[m]) vn[idx[m]] = cscale(x[m], sc);
+}
+
+using step_fn = void (*)(cd *, size_t, size_t, int, int, const cd *, cd *,
+                         cd *, cd *, cd *, cd *, SysState *, double *, int *,
+                         double, int *);
+
+static step_fn step_kernel(size_t N) {
+    static const step_fn tab[kCgsPer] = {
+        arnoldi_step_kernel<1>, arnoldi_step_kernel<2>, arnoldi_step_kernel<3>,
+        arnoldi_step_kernel<4>, arnoldi_step_kernel<5>, arnoldi_step_kernel<6>,
+        arnoldi_step_kernel<7>, arnoldi_step_kernel<8>};
+    return tab[(N + kCgsThreads - 1) / kCgsThreads - 1];
 }
 
 // Start of a restart cycle: inv_s = 1/||M r_s||, S_s = [||M r_s||, 0, ...]
 // (the first column of the Arnoldi basis is M r_s * inv_s).
 __global__ void cycle_start_kernel(int nsys, int R, const cd *nrm,
-                                   double *inv, cd *S) {
+                                   double *inv, cd *S, int *still) {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < (size_t)R) still[i] = 0;
     if (i >= (size_t)nsys * (R + 1)) return;
     const int s = (int)(i / (R + 1)), q = (int)(i % (R + 1));
     const double t = sqrt(nrm[s].r);
@@ -861,7 +925,7 @@ static Work carve(void *base, int nsys, size_t N, int restart, int Nx) {
     w.inv = (double *)take(sizeof(double) * nsys);
     w.st = (SysState *)take(sizeof(SysState) * nsys);
     w.active = (int *)take(sizeof(int) * nsys);
-    w.still = (int *)take(sizeof(int));
+    w.still = (int *)take(sizeof(int) * R);   // one word per column
     w.bytes = off;
     return w;
 }
@@ -1447,7 +1511,7 @@ extern "C" int dyf_bragg_solve(const dyf_bragg_args *a, const double *s_dev,
         if ((rc = apply_precon(k, W.pinv, W.r, W.w, stream))) return rc;
         if ((rc = norm_into(W.w, W.nrm, W.active))) return rc;
         cycle_start_kernel<<<sgrid, 256, 0, stream>>>(nsys, R, W.nrm, W.inv,
-                                                      W.S);
+                                                      W.S, W.still);
         DYF_LAUNCHED("cycle_start_kernel");
         scale_copy_kernel<<<vgrid, 256, 0, stream>>>(W.w, N, W.V, Vsys, W.inv,
                                                      N, W.active);
@@ -1463,10 +1527,10 @@ extern "C" int dyf_bragg_solve(const dyf_bragg_args *a, const double *s_dev,
             // h0 = ||w||; CGS2 against v_0..v_col
             if (N <= (size_t)kCgsMaxN) {
                 const size_t lds = sizeof(double) * 10 * (size_t)(col + 1);
-                cgs2_kernel<<<nsys, kCgsThreads, std::max(lds, sizeof(double) * 4),
-                              stream>>>(W.V, Vsys, N, col + 1, W.w, W.nrm0,
-                                        W.h1, W.h2, R + 1, W.nrm, W.active);
-                DYF_LAUNCHED("cgs2_kernel");
+                step_kernel(N)<<<nsys, kCgsThreads, lds, stream>>>(
+                    W.V, Vsys, N, R, col, W.w, W.h1, W.h2, W.H, W.G, W.S,
+                    W.st, W.inv, W.active, eps, W.still + col);
+                DYF_LAUNCHED("arnoldi_step_kernel");
             } else {
                 rc = dots(W.w, N, 0, 1, W.w, N, N, W.nrm0, 1, W.part, nsys,
                           W.active, stream);
@@ -1484,19 +1548,17 @@ extern "C" int dyf_bragg_solve(const dyf_bragg_args *a, const double *s_dev,
                 rc = dots(W.w, N, 0, 1, W.w, N, N, W.nrm, 1, W.part, nsys,
                           W.active, stream);
                 if (rc) return rc;
+                arnoldi_finish_kernel<<<(nsys + 63) / 64, 64, 0, stream>>>(
+                    nsys, R, col, W.H, W.G, W.S, W.h1, W.h2, W.nrm, W.nrm0,
+                    W.st, W.inv, W.active, eps, W.still + col);
+                DYF_LAUNCHED("arnoldi_finish_kernel");
+                // v_{col+1} = w / h1 for the systems still iterating
+                scale_copy_kernel<<<vgrid, 256, 0, stream>>>(
+                    W.w, N, W.V + (size_t)(col + 1) * N, Vsys, W.inv, N,
+                    W.active);
+                DYF_LAUNCHED("scale_copy_kernel");
             }
-            zero_int_kernel<<<1, 1, 0, stream>>>(W.still);
-            DYF_LAUNCHED("zero_int_kernel");
-            arnoldi_finish_kernel<<<(nsys + 63) / 64, 64, 0, stream>>>(
-                nsys, R, col, W.H, W.G, W.S, W.h1, W.h2, W.nrm, W.nrm0, W.st,
-                W.inv, W.active, eps, W.still);
-            DYF_LAUNCHED("arnoldi_finish_kernel");
-            // v_{col+1} = w / h1 for the systems still iterating
-            scale_copy_kernel<<<vgrid, 256, 0, stream>>>(
-                W.w, N, W.V + (size_t)(col + 1) * N, Vsys, W.inv, N,
-                W.active);
-            DYF_LAUNCHED("scale_copy_kernel");
-            DYF_HIP(hipMemcpyAsync(&still_h[col & 1], W.still, sizeof(int),
+            DYF_HIP(hipMemcpyAsync(&still_h[col & 1], W.still + col, sizeof(int),
                                    hipMemcpyDeviceToHost, stream));
             DYF_HIP(hipEventRecord(still_ev[col & 1], stream));
             if (col > 0) {
