@@ -203,8 +203,8 @@ __device__ __forceinline__ cd wave_fetch(cd v, int src4) {
             __longlong_as_double(((long long)ih << 32) | (unsigned)il)};
 }
 
-template <int C>
-__global__ __launch_bounds__(kWave) void bragg_march_wave_kernel(
+template <int C, int D = (C <= 10 ? 3 : 2)>
+__global__ __launch_bounds__(kWave, (C <= 11 ? 2 : 1)) void bragg_march_wave_kernel(
     BraggK k, const double *__restrict__ s_dev, const cd *__restrict__ E,
     const cd *__restrict__ inc, const cd *__restrict__ vin, size_t vin_stride,
     cd *__restrict__ out, size_t out_stride, double out_scale,
@@ -227,7 +227,7 @@ __global__ __launch_bounds__(kWave) void bragg_march_wave_kernel(
     const int x0 = (real ? lane : Lu - 1) * C;   // idle lanes read in bounds
     const int srcX = (real ? (lane == 0 ? Lu - 1 : lane - 1) : lane) << 2;
     const int srcY = (real ? (lane == Lu - 1 ? 0 : lane + 1) : lane) << 2;
-    cd X[C], Y[C], en[C];
+    cd X[C], Y[C];
     const cd *vj = vin ? vin + (size_t)s * vin_stride + (size_t)j * Nx : nullptr;
 #pragma unroll
     for (int i = 0; i < C; ++i) {
@@ -236,17 +236,16 @@ __global__ __launch_bounds__(kWave) void bragg_march_wave_kernel(
     }
     const size_t plane = (size_t)k.tpl * Nx;
     const cd *Ej = E + (size_t)j * Nx + x0;
+    // e^{iH} slices run D steps ahead in D register sets with fixed roles
+    // (the loop is unrolled by D), so a slice's load has D - 1 steps of
+    // work to land in and no register copy waits for it.
+    cd eb[D][C];
+    auto load = [&](cd (&e)[C], int t) {
+        const cd *Et = Ej + (size_t)(t < nst ? t : nst - 1) * plane;
 #pragma unroll
-    for (int i = 0; i < C; ++i) en[i] = Ej[i];
-    __syncthreads();
-    for (int step = 0; step < nst; ++step) {
-        cd ec[C];
-#pragma unroll
-        for (int i = 0; i < C; ++i) ec[i] = en[i];
-        const int sn = step + 1 < nst ? step + 1 : step;
-        const cd *En = Ej + (size_t)sn * plane;
-#pragma unroll
-        for (int i = 0; i < C; ++i) en[i] = En[i];
+        for (int i = 0; i < C; ++i) e[i] = Et[i];
+    };
+    auto advance = [&](const cd (&ec)[C], int step) {
         const cd ck = coef[step], bk = coef[nst + step];
         cd xn[C], yn[C];
 #pragma unroll
@@ -265,7 +264,21 @@ __global__ __launch_bounds__(kWave) void bragg_march_wave_kernel(
         Y[C - 1] = wave_fetch(yn[0], srcY);
 #pragma unroll
         for (int i = 0; i < C - 1; ++i) Y[i] = yn[i + 1];
+    };
+#pragma unroll
+    for (int d = 0; d < D; ++d) load(eb[d], d);
+    __syncthreads();
+    int step = 0;
+    for (; step + D <= nst; step += D) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            advance(eb[d], step + d);
+            load(eb[d], step + d + D);
+        }
     }
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+        if (step + d < nst) advance(eb[d], step + d);
     const double egr = k.s0r * k.ds, egi = (k.s0i + alpha) * k.ds;
     const cd fin = cscale(cexp(-(double)nst * egr, -(double)nst * egi),
                           out_scale);
