@@ -289,6 +289,141 @@ __global__ __launch_bounds__(kWave, (C <= 11 ? 2 : 1)) void bragg_march_wave_ker
     }
 }
 
+// The same march with each row split over the two wavefronts of a
+// workgroup, for launches with too few rows to load every SIMD evenly
+// (one wavefront per row leaves some SIMDs with two rows and others with
+// one).  Wavefront w owns columns [w Nx/2, (w+1) Nx/2) in Lo lanes and
+// marches Hc = Hl C further columns on each side in Hl otherwise idle
+// lanes.  Column x after t steps depends only on columns x - t .. x + t,
+// so the halos absorb the wrong values that enter at the wavefront's
+// outer lanes for up to Hc steps; every P <= Hc steps the halos are
+// refreshed from the other wavefront's owned edge columns through LDS
+// (one barrier, buffers alternating between refreshes).  The owned
+// columns see exactly the arithmetic of the one-wavefront kernel.
+template <int C, int D = 3>
+__global__ __launch_bounds__(2 * kWave, (C <= 6 ? 3 : 2)) void bragg_march_halo_kernel(
+    BraggK k, const double *__restrict__ s_dev, const cd *__restrict__ E,
+    const cd *__restrict__ inc, const cd *__restrict__ vin, size_t vin_stride,
+    cd *__restrict__ out, size_t out_stride, double out_scale,
+    const int *__restrict__ active) {
+    extern __shared__ cd coef[];   // c_k, b_k, then [2][wave][edge][X,Y][Hc]
+    const int s = blockIdx.x / k.nyc, j = blockIdx.x % k.nyc;
+    if (active && !active[s]) return;
+    E += table_offset(k, s);
+    const int Nx = k.Nx, half = Nx / 2, lane = threadIdx.x % kWave;
+    const int wv = threadIdx.x / kWave;
+    const int Lo = half / C, Hl = (kWave - Lo) / 2, Hc = Hl * C;
+    const int P = Hc / D * D;                 // steps between refreshes
+    const int nst = k.Nz - 1;
+    const double alpha = -2.0 * kPi * s_dev[s];
+    const double lr = 2.0 * k.s0r * k.ds, li = (2.0 * k.s0i + alpha) * k.ds;
+    for (int t = threadIdx.x; t < nst; t += 2 * kWave) {
+        const cd f = cexp(-(double)t * lr, -(double)t * li);   // (e0 eg)^-t
+        const cd g = cexp((double)t * lr, (double)t * li);     // (e0 eg)^t
+        coef[t] = cmul(cd{k.ds * k.sbr, k.ds * k.sbi}, f);
+        coef[nst + t] = cmul(cd{-k.ds * k.shr, -k.ds * k.shi}, g);
+    }
+    cd *edge = coef + 2 * nst;
+    const int rel = lane - Hl;                // owned lane index if in [0, Lo)
+    const int x0 = ((wv * half + rel * C) % Nx + Nx) % Nx;
+    const int srcX = (lane == 0 ? kWave - 1 : lane - 1) << 2;
+    const int srcY = (lane == kWave - 1 ? 0 : lane + 1) << 2;
+    cd X[C], Y[C];
+    const cd *vj = vin ? vin + (size_t)s * vin_stride + (size_t)j * Nx : nullptr;
+#pragma unroll
+    for (int i = 0; i < C; ++i) {
+        X[i] = inc ? inc[x0 + i] : cd{0.0, 0.0};
+        Y[i] = vj ? vj[x0 + i] : cd{0.0, 0.0};
+    }
+    const size_t plane = (size_t)k.tpl * Nx;
+    const cd *Ej = E + (size_t)j * Nx + x0;
+    cd eb[D][C];
+    auto load = [&](cd (&e)[C], int t) {
+        const cd *Et = Ej + (size_t)(t < nst ? t : nst - 1) * plane;
+#pragma unroll
+        for (int i = 0; i < C; ++i) e[i] = Et[i];
+    };
+    auto advance = [&](const cd (&ec)[C], int step) {
+        const cd ck = coef[step], bk = coef[nst + step];
+        cd xn[C], yn[C];
+#pragma unroll
+        for (int i = 0; i < C; ++i) {
+            const cd e = ec[i];
+            const cd u = cmul(ck, e);              // c_k e^{+iH}
+            const cd w = cmul(bk, cd{e.r, -e.i});  // b_k e^{-iH}
+            xn[i] = cd{fma(u.r, Y[i].r, fma(-u.i, Y[i].i, X[i].r)),
+                       fma(u.r, Y[i].i, fma(u.i, Y[i].r, X[i].i))};
+            yn[i] = cd{fma(w.r, X[i].r, fma(-w.i, X[i].i, Y[i].r)),
+                       fma(w.r, X[i].i, fma(w.i, X[i].r, Y[i].i))};
+        }
+        X[0] = wave_fetch(xn[C - 1], srcX);
+#pragma unroll
+        for (int i = 1; i < C; ++i) X[i] = xn[i - 1];
+        Y[C - 1] = wave_fetch(yn[0], srcY);
+#pragma unroll
+        for (int i = 0; i < C - 1; ++i) Y[i] = yn[i + 1];
+    };
+    // edge[((par * 2 + wave) * 2 + side) * 2 + field][Hc]
+    auto slot = [&](int par, int w, int side, int field) {
+        return edge + (size_t)(((par * 2 + w) * 2 + side) * 2 + field) * Hc;
+    };
+    int par = 0;
+    auto refresh = [&]() {
+        if (rel >= 0 && rel < Hl) {           // first Hc owned columns
+#pragma unroll
+            for (int i = 0; i < C; ++i)
+                slot(par, wv, 0, 0)[rel * C + i] = X[i],
+                slot(par, wv, 0, 1)[rel * C + i] = Y[i];
+        }
+        if (rel >= Lo - Hl && rel < Lo) {     // last Hc owned columns
+            const int r = rel - (Lo - Hl);
+#pragma unroll
+            for (int i = 0; i < C; ++i)
+                slot(par, wv, 1, 0)[r * C + i] = X[i],
+                slot(par, wv, 1, 1)[r * C + i] = Y[i];
+        }
+        __syncthreads();
+        if (lane < Hl) {                      // left halo: other's last
+#pragma unroll
+            for (int i = 0; i < C; ++i)
+                X[i] = slot(par, 1 - wv, 1, 0)[lane * C + i],
+                Y[i] = slot(par, 1 - wv, 1, 1)[lane * C + i];
+        } else if (rel >= Lo && rel < Lo + Hl) {   // right halo: other's first
+            const int r = rel - Lo;
+#pragma unroll
+            for (int i = 0; i < C; ++i)
+                X[i] = slot(par, 1 - wv, 0, 0)[r * C + i],
+                Y[i] = slot(par, 1 - wv, 0, 1)[r * C + i];
+        }
+        par ^= 1;
+    };
+#pragma unroll
+    for (int d = 0; d < D; ++d) load(eb[d], d);
+    __syncthreads();
+    int step = 0, since = 0;
+    for (; step + D <= nst; step += D) {
+        if (since == P) refresh(), since = 0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            advance(eb[d], step + d);
+            load(eb[d], step + d + D);
+        }
+        since += D;
+    }
+    if (step < nst && since == P) refresh();
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+        if (step + d < nst) advance(eb[d], step + d);
+    const double egr = k.s0r * k.ds, egi = (k.s0i + alpha) * k.ds;
+    const cd fin = cscale(cexp(-(double)nst * egr, -(double)nst * egi),
+                          out_scale);
+    if (rel >= 0 && rel < Lo) {
+        cd *o = out + (size_t)s * out_stride + (size_t)j * Nx + x0;
+#pragma unroll
+        for (int i = 0; i < C; ++i) o[i] = cmul(Y[i], fin);
+    }
+}
+
 using MarchFn = void (*)(BraggK, const double *, const cd *, const cd *,
                          const cd *, size_t, cd *, size_t, double,
                          const int *);
@@ -310,6 +445,33 @@ static int wave_columns(int Nx) {
     return 0;
 }
 
+template <int... Cs>
+static MarchFn halo_fn(int C) {
+    MarchFn f = nullptr;
+    ((C == Cs ? (f = bragg_march_halo_kernel<Cs>, 0) : 0), ...);
+    return f;
+}
+
+// Columns per lane of the two-wavefront march: the smallest C <= 16 with
+// C | Nx/2 that leaves at least one spare lane on each side of the Nx/2/C
+// owned lanes, a halo Hc = C (64 - Nx/2/C)/2 of at least 3 columns (one
+// unrolled group of steps between refreshes) and no wider than the owned
+// part; 0 if none (or Nx is odd).
+static int halo_columns(int Nx) {
+    if (Nx % 2) return 0;
+    const int half = Nx / 2;
+    for (int c = 1; c <= kWaveMaxC; ++c) {
+        if (half % c) continue;
+        const int Lo = half / c, Hl = (kWave - Lo) / 2;
+        if (Hl >= 1 && Hl * c >= 3 && Hl <= Lo) return c;
+    }
+    return 0;
+}
+
+// Below this many rows (system-planes) a march launch splits each row over
+// two wavefronts: one wavefront per row would load the SIMDs unevenly.
+constexpr long long kHaloRows = 4096;
+
 template <int... Ms>
 static MarchFn march_fn(int M) {
     MarchFn f = nullptr;
@@ -323,6 +485,20 @@ static int launch_march(const BraggK &k, const double *s_dev, const cd *E,
                         const int *active, hipStream_t stream) {
     const int C = wave_columns(k.Nx);
     const size_t coef = (size_t)2 * std::max(k.Nz - 1, 1) * sizeof(cd);
+    const int Ch = halo_columns(k.Nx);
+    if (Ch && (long long)k.nsys * k.nyc < kHaloRows) {
+        const int Hc = (kWave - k.Nx / 2 / Ch) / 2 * Ch;
+        const size_t lds = coef + (size_t)16 * Hc * sizeof(cd);
+        if (lds <= 64 * 1024) {
+            MarchFn hf = halo_fn<1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14,
+                                 15, 16>(Ch);
+            hipLaunchKernelGGL(hf, dim3(k.nsys * k.nyc), dim3(2 * kWave), lds,
+                               stream, k, s_dev, E, inc, vin, vin_stride, out,
+                               out_stride, out_scale, active);
+            DYF_LAUNCHED("bragg_march_halo_kernel");
+            return DYF_OK;
+        }
+    }
     if (C && coef <= 64 * 1024) {
         MarchFn wf = wave_fn<1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15,
                              16>(C);

@@ -30,9 +30,10 @@ def _rel(a, b):
                  / np.abs(np.asarray(b)).max())
 
 
-# Nx = 324 = 54 lanes x 6 columns takes the wavefront march; Nx = 334 =
-# 2 x 167 has no column count <= 16 that divides it into <= 64 lanes and
-# takes the workgroup-per-plane march.
+# Nx = 324 takes the two-wavefront march (each wavefront owns 162 = 54
+# lanes x 3 columns); Nx = 334 = 2 x 167 has no column count <= 16 that
+# divides it (or its half) into <= 64 lanes and takes the
+# workgroup-per-plane march.
 @pytest.mark.parametrize("half_x", [2e-6, 3e-6],
                          ids=["wavefront-Nx324", "workgroup-Nx334"])
 def test_phase_march_and_preconditioner_match_oracle(dev, diamond400, g, rng,
@@ -65,6 +66,27 @@ def test_phase_march_and_preconditioner_match_oracle(dev, diamond400, g, rng,
         want_w = np.fft.ifft(np.fft.fft(v[j], axis=-1) / T22s, axis=-1)
         got_w = BS.precondition(cfg, diamond400, [sd], v[j][None])[0]
         assert _rel(got_w, want_w) < 1e-12
+
+
+def test_one_and_two_wavefront_marches_agree(dev, diamond400, g, rng):
+    # 41 angles x 100 planes = 4100 rows take the one-wavefront march (at
+    # least 4096 rows); one angle (100 rows) takes the two-wavefront march,
+    # whose owned columns do the same arithmetic
+    cfg = BraggConfig(diamond400, t_crystal=6e-6, dx=0.2e-6, half_x=2e-6)
+    seg = threading_segment("Edge", +1, diamond400.b_mag, float(g[2]))
+    ys = (np.arange(100) - 50) * 0.05e-6
+    E = BS.chunk_phase(seg, cfg, diamond400, g, ys)
+    s = deviation_from_urad(np.linspace(-20.0, 20.0, 41), diamond400)
+    v = rng.normal(size=(41, 100, cfg.Nx)) + 1j * rng.normal(
+        size=(41, 100, cfg.Nx))
+    inc = cfg.incident(n=1)[0]
+    all_h = BS.march(cfg, diamond400, s, E, Dg0=v).get()
+    all_p = BS.march(cfg, diamond400, s, E, incident=inc).get()
+    for j in (0, 17, 40):
+        one_h = BS.march(cfg, diamond400, [s[j]], E, Dg0=v[j][None]).get()[0]
+        one_p = BS.march(cfg, diamond400, [s[j]], E, incident=inc).get()[0]
+        assert _rel(one_h, all_h[j]) < 1e-13
+        assert _rel(one_p, all_p[j]) < 1e-13
 
 
 def test_phase_table_integer_and_fractional_winding(dev, diamond400, g):
