@@ -1004,12 +1004,59 @@ static step_fn step_kernel(size_t N) {
 __global__ void cycle_start_kernel(int nsys, int R, const cd *nrm,
                                    double *inv, cd *S, int *still) {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (i < (size_t)R) still[i] = 0;
+    if (i <= (size_t)R) still[i] = 0;   // R per-column words, then `any`
     if (i >= (size_t)nsys * (R + 1)) return;
     const int s = (int)(i / (R + 1)), q = (int)(i % (R + 1));
     const double t = sqrt(nrm[s].r);
     S[i] = q == 0 ? cd{t, 0.0} : cd{0.0, 0.0};
     if (q == 0) inv[s] = t > 0.0 ? 1.0 / t : 0.0;
+}
+
+// SciPy's gmres set-up for every system from ||b||^2, ||M b||^2 and
+// ||r0||^2, and the first cycle's active flags.
+__global__ void gmres_setup_kernel(int nsys, const cd *nb, const cd *nmb,
+                                   const cd *nr, double tol, int maxiter,
+                                   SysState *st, int *active) {
+    const int s = blockIdx.x * 64 + threadIdx.x;
+    if (s >= nsys) return;
+    SysState q{};
+    q.bnrm2 = sqrt(nb[s].r);
+    q.atol = tol * q.bnrm2;
+    q.pmf = 1.0;
+    q.code = maxiter;
+    if (q.bnrm2 == 0.0) q.done = 1, q.code = 0;   // SciPy returns b (= 0)
+    const double Mb = sqrt(nmb[s].r);
+    q.ptol = q.bnrm2 > 0.0 ? Mb * fmin(1.0, q.atol / q.bnrm2) : 0.0;
+    q.rnorm = sqrt(nr[s].r);
+    if (!q.done && q.rnorm < q.atol) q.done = 1, q.code = 0;
+    st[s] = q;
+    active[s] = !q.done;
+}
+
+// End of a restart cycle for every system not yet done: SciPy's
+// convergence test on the true residual ||r||^2 (nr), the breakdown exit
+// and the update of the inner tolerance; then the next cycle's flags.
+// `any` (cleared at the cycle's start) is set while a system continues.
+__global__ void gmres_cycle_end_kernel(int nsys, const cd *nr, double eps,
+                                       SysState *st, int *active, int *any) {
+    const int s = blockIdx.x * 64 + threadIdx.x;
+    if (s >= nsys) return;
+    SysState &q = st[s];
+    if (!q.done) {
+        q.rnorm = sqrt(nr[s].r);
+        if (q.rnorm <= q.atol) {
+            q.done = 1, q.code = 0;
+        } else if (q.breakdown) {
+            q.done = 1;
+        } else {
+            q.pmf = q.presid <= q.ptol ? fmax(eps, 0.25 * q.pmf)
+                                       : fmin(1.0, 1.5 * q.pmf);
+            q.ptol = q.presid * fmin(q.pmf, q.atol / q.rnorm);
+        }
+    }
+    q.col = 0, q.breakdown = 0, q.inner_done = 0;
+    active[s] = !q.done;
+    if (!q.done) atomicOr(any, 1);
 }
 
 // After the inner loop (which clears a system's flag as its Arnoldi
@@ -1114,7 +1161,7 @@ static Work carve(void *base, int nsys, size_t N, int restart, int Nx) {
     w.inv = (double *)take(sizeof(double) * nsys);
     w.st = (SysState *)take(sizeof(SysState) * nsys);
     w.active = (int *)take(sizeof(int) * nsys);
-    w.still = (int *)take(sizeof(int) * R);   // one word per column
+    w.still = (int *)take(sizeof(int) * (R + 1));   // per column, `any`
     w.bytes = off;
     return w;
 }
@@ -1595,32 +1642,36 @@ extern "C" int dyf_bragg_solve(const dyf_bragg_args *a, const double *s_dev,
     const size_t Vsys = (size_t)(R + 1) * N;
     const dim3 vgrid((unsigned)((N + 255) / 256), nsys);
 
-    std::vector<SysState> st(nsys);
-    std::vector<int> act(nsys, 1);
-    // Host<->device traffic is a few small copies per restart cycle, each
-    // group followed by one synchronisation; the host vectors are only
-    // written after such a synchronisation, so the asynchronous uploads
-    // never see them change.
-    auto push_state = [&]() -> int {
-        DYF_HIP(hipMemcpyAsync(W.st, st.data(), sizeof(SysState) * nsys,
-                               hipMemcpyHostToDevice, stream));
-        DYF_HIP(hipMemcpyAsync(W.active, act.data(), sizeof(int) * nsys,
-                               hipMemcpyHostToDevice, stream));
-        return DYF_OK;
-    };
+    // The per-system GMRES state lives on the device (set-up, inner-loop
+    // and end-of-cycle kernels); the host reads one word per Arnoldi step
+    // (pipelined, below) and one per restart cycle, and the states once at
+    // the end, through page-locked buffers.
+    static thread_local int *flags_h = nullptr;       // [2] still, [2] any
+    static thread_local hipEvent_t still_ev[2] = {nullptr, nullptr};
+    static thread_local SysState *st_h = nullptr;
+    static thread_local int st_cap = 0;
+    if (!flags_h) {
+        DYF_HIP(hipHostMalloc((void **)&flags_h, 4 * sizeof(int),
+                              hipHostMallocDefault));
+        for (int e = 0; e < 2; ++e)
+            DYF_HIP(hipEventCreateWithFlags(&still_ev[e],
+                                            hipEventDisableTiming));
+    }
+    if (st_cap < nsys) {
+        if (st_h) DYF_HIP(hipHostFree(st_h));
+        st_h = nullptr, st_cap = 0;
+        DYF_HIP(hipHostMalloc((void **)&st_h, sizeof(SysState) * nsys,
+                              hipHostMallocDefault));
+        st_cap = nsys;
+    }
+    int *still_h = flags_h, *any_h = flags_h + 2;
     auto norm_into = [&](const cd *v, cd *out, const int *active) -> int {
         return dots(v, N, 0, 1, v, N, N, out, 1, W.part, nsys, active,
                     stream);
     };
-    auto fetch = [&](std::vector<cd> &h, const cd *d) -> int {
-        DYF_HIP(hipMemcpyAsync(h.data(), d, sizeof(cd) * nsys,
-                               hipMemcpyDeviceToHost, stream));
-        return DYF_OK;
-    };
-    std::vector<cd> nb(nsys), nmb(nsys), nr(nsys);
 
     // preconditioner symbol for every system; b = -march(incident, 0);
-    // ||b||, ||M b|| and ||r|| (r = b - A x0, or b) in one round trip
+    // ||b||, ||M b|| and ||r|| (r = b - A x0, or b) into the set-up
     bragg_symbol_kernel<<<nsys, 256, 0, stream>>>(k, s_dev, W.pinv);
     DYF_LAUNCHED("bragg_symbol_kernel");
     rc = launch_march(k, s_dev, E, (const cd *)incident, nullptr, 0, W.b, N,
@@ -1643,59 +1694,27 @@ extern "C" int dyf_bragg_solve(const dyf_bragg_args *a, const double *s_dev,
         sub_kernel<<<vgrid, 256, 0, stream>>>(W.b, W.av, W.r, N, nullptr);
         DYF_LAUNCHED("sub_kernel");
         if ((rc = norm_into(W.r, W.h2, nullptr))) return rc;
-        if ((rc = fetch(nr, W.h2))) return rc;
     } else {
         DYF_HIP(hipMemcpyAsync(W.r, W.b, sizeof(cd) * nsys * N,
                                hipMemcpyDeviceToDevice, stream));
     }
-    if ((rc = fetch(nb, W.nrm)) || (rc = fetch(nmb, W.nrm0))) return rc;
-    DYF_HIP(hipStreamSynchronize(stream));
-    if (!x0) nr = nb;
-    for (int s = 0; s < nsys; ++s) {
-        st[s] = SysState{};
-        st[s].bnrm2 = sqrt(nb[s].r);
-        st[s].atol = opt->tol * st[s].bnrm2;
-        st[s].pmf = 1.0;
-        st[s].code = opt->maxiter;
-        if (st[s].bnrm2 == 0.0) {   // SciPy returns b (= 0) at once
-            st[s].done = 1, st[s].code = 0, act[s] = 0;
-        }
-        const double Mb = sqrt(nmb[s].r);
-        st[s].ptol = st[s].bnrm2 > 0.0
-                         ? Mb * std::min(1.0, st[s].atol / st[s].bnrm2)
-                         : 0.0;
-        st[s].rnorm = sqrt(nr[s].r);
-        if (!st[s].done && st[s].rnorm < st[s].atol) {
-            st[s].done = 1, st[s].code = 0, act[s] = 0;
-        }
-    }
+    const unsigned tgrid = (unsigned)((nsys + 63) / 64);
+    gmres_setup_kernel<<<tgrid, 64, 0, stream>>>(
+        nsys, W.nrm, W.nrm0, x0 ? W.h2 : W.nrm, opt->tol, opt->maxiter, W.st,
+        W.active);
+    DYF_LAUNCHED("gmres_setup_kernel");
 
     // The inner loop reads each step's "still iterating" word one step
     // late: the copy of step c lands in a page-locked slot with an event,
     // and the host waits on step c-1's event only after enqueueing step
     // c, so the device never idles on the host.  A step enqueued after
     // every system stopped is masked out by the active flags (only the
-    // preconditioner's FFTs run on it).
-    static thread_local int *still_h = nullptr;
-    static thread_local hipEvent_t still_ev[2] = {nullptr, nullptr};
-    if (!still_h) {
-        DYF_HIP(hipHostMalloc((void **)&still_h, 2 * sizeof(int),
-                              hipHostMallocDefault));
-        for (int e = 0; e < 2; ++e)
-            DYF_HIP(hipEventCreateWithFlags(&still_ev[e],
-                                            hipEventDisableTiming));
-    }
+    // preconditioner's FFTs run on it).  The first cycle starts without a
+    // look at the set-up: if no system needs it, it is masked out the
+    // same way and its end-of-cycle word ends the loop.
     const unsigned sgrid = (unsigned)(((size_t)nsys * (R + 1) + 255) / 256);
 
     for (int cycle = 0; cycle < opt->maxiter; ++cycle) {
-        int any = 0;
-        for (int s = 0; s < nsys; ++s) {
-            act[s] = !st[s].done;
-            st[s].col = 0, st[s].breakdown = 0, st[s].inner_done = 0;
-            any |= act[s];
-        }
-        if (!any) break;
-        if ((rc = push_state())) return rc;
         // v0 = M r / ||M r||, S = [||M r||, 0, ...]
         if ((rc = apply_precon(k, W.pinv, W.r, W.w, stream))) return rc;
         if ((rc = norm_into(W.w, W.nrm, W.active))) return rc;
@@ -1772,32 +1791,24 @@ extern "C" int dyf_bragg_solve(const dyf_bragg_args *a, const double *s_dev,
         sub_kernel<<<vgrid, 256, 0, stream>>>(W.b, W.av, W.r, N, W.active);
         DYF_LAUNCHED("sub_kernel");
         if ((rc = norm_into(W.r, W.nrm, W.active))) return rc;
-        DYF_HIP(hipMemcpyAsync(st.data(), W.st, sizeof(SysState) * nsys,
-                               hipMemcpyDeviceToHost, stream));
-        if ((rc = fetch(nr, W.nrm))) return rc;
-        DYF_HIP(hipStreamSynchronize(stream));
-        for (int s = 0; s < nsys; ++s) {
-            if (st[s].done) continue;
-            SysState &q = st[s];
-            q.rnorm = sqrt(nr[s].r);
-            if (q.rnorm <= q.atol) {
-                q.done = 1, q.code = 0;
-            } else if (q.breakdown) {
-                q.done = 1;
-            } else {
-                q.pmf = q.presid <= q.ptol ? std::max(eps, 0.25 * q.pmf)
-                                           : std::min(1.0, 1.5 * q.pmf);
-                q.ptol = q.presid * std::min(q.pmf, q.atol / q.rnorm);
-            }
-        }
+        gmres_cycle_end_kernel<<<tgrid, 64, 0, stream>>>(
+            nsys, W.nrm, eps, W.st, W.active, W.still + R);
+        DYF_LAUNCHED("gmres_cycle_end_kernel");
         // reset the Hessenberg for the next cycle
         DYF_HIP(hipMemsetAsync(W.H, 0, sizeof(cd) * nsys * R * (R + 1), stream));
+        DYF_HIP(hipMemcpyAsync(any_h, W.still + R, sizeof(int),
+                               hipMemcpyDeviceToHost, stream));
+        DYF_HIP(hipStreamSynchronize(stream));
+        if (!*any_h) break;
     }
-    for (int s = 0; s < nsys; ++s) {
-        info[s].residual = st[s].bnrm2 > 0.0 ? st[s].rnorm / st[s].bnrm2 : 0.0;
-        info[s].iterations = st[s].iters;
-        info[s].code = st[s].rnorm <= st[s].atol ? 0 : opt->maxiter;
-    }
+    DYF_HIP(hipMemcpyAsync(st_h, W.st, sizeof(SysState) * nsys,
+                           hipMemcpyDeviceToHost, stream));
     DYF_HIP(hipStreamSynchronize(stream));
+    for (int s = 0; s < nsys; ++s) {
+        const SysState &q = st_h[s];
+        info[s].residual = q.bnrm2 > 0.0 ? q.rnorm / q.bnrm2 : 0.0;
+        info[s].iterations = q.iters;
+        info[s].code = q.rnorm <= q.atol ? 0 : opt->maxiter;
+    }
     return DYF_OK;
 }
