@@ -1477,6 +1477,271 @@ __global__ __launch_bounds__(kWave) void bragg_fixed_point_wave_kernel(
     if (lane == 0) info[blockIdx.x] = dyf_fixed_info{res, iters, conv, };
 }
 
+// The same solve with each row split over W wavefronts of a workgroup,
+// for launches with few rows (one wavefront per row leaves most SIMDs idle
+// and every depth step waiting on memory).  Wavefront w owns columns
+// [w Nx/W, (w+1) Nx/W) on its last Lo lanes and marches the Hc = Hl C
+// columns to their left in its first Hl lanes.  Both sweeps roll the
+// marched field one column towards +x per depth step, so wrong values
+// enter only at lane 0 and move right one column per step: the owned
+// columns stay exact for Hc steps, and every P <= Hc steps the halo takes
+// the left neighbour's current values of those columns through LDS (one
+// barrier, buffers alternating).  The coupling terms come from the stored
+// fields at each lane's own columns, exact in the halo too.  Fields are
+// stored (row, z, x); the e^{iH} and field values of the next D depths are
+// in flight in D register sets with fixed roles (steps unrolled by D).
+// Stores carry no branch: halo lanes (and the z = 0 row of the backward
+// sweep, written after the convergence test) store to a per-lane slot of
+// `trash`.  A branch around them made the compiler wait for the loads
+// of later depths too (stores count in the same counter).  The sweeps
+// read columns other wavefronts of the workgroup stored; the barrier
+// between sweeps orders those global stores before the loads (the
+// wavefronts of a workgroup share one CU and its vector L1).
+template <int C, int W, int D = (8 / C > 2 ? 8 / C : 2)>
+__global__ __launch_bounds__(kWave * W) void bragg_fixed_point_halo_kernel(
+    BraggK k, const double *__restrict__ s_dev, const cd *__restrict__ E,
+    const cd *__restrict__ inc, int n_iter, double tol, cd *__restrict__ D0,
+    cd *__restrict__ Dg, cd *__restrict__ out, dyf_fixed_info *__restrict__ info,
+    cd *__restrict__ trash) {
+    __shared__ cd xch[2][W][kWave / 2 * C];      // [parity][wave][Hc]
+    __shared__ double red[2][W];
+    const int sys = blockIdx.x / k.nyc, j = blockIdx.x % k.nyc;
+    const int Nx = k.Nx, Nz = k.Nz, lane = threadIdx.x % kWave;
+    const int wv = threadIdx.x / kWave, wl = wv == 0 ? W - 1 : wv - 1;
+    const int Nw = Nx / W, Lo = Nw / C, Hl = kWave - Lo, Hc = Hl * C;
+    const int P = Hc / D * D;                  // steps between refreshes
+    const int rel = lane - Hl;
+    const bool own = rel >= 0;
+    const int x0 = ((wv * Nw + rel * C) % Nx + Nx) % Nx;
+    const int src = (lane == 0 ? kWave - 1 : lane - 1) << 2;
+    const cd *Es = E + table_offset(k, sys) + (size_t)j * Nx + x0;
+    const size_t zs = (size_t)k.tpl * Nx;             // e^{iH} depth stride
+    cd *d0 = D0 + (size_t)blockIdx.x * Nz * Nx + x0;
+    cd *dg = Dg + (size_t)blockIdx.x * Nz * Nx + x0;
+    // this lane's slot of `trash` (halo lanes are lanes < Hl <= 32)
+    cd *tr = trash + ((size_t)blockIdx.x * W + wv) * (kWave / 2 * C) +
+             (own ? 0 : lane * C);
+    const double alpha = -2.0 * kPi * s_dev[sys];
+    const cd e0 = cexp(k.s0r * k.ds, k.s0i * k.ds);
+    const cd eg = cexp(k.s0r * k.ds, (k.s0i + alpha) * k.ds);
+    const cd sb = {k.sbr, k.sbi}, sh = {k.shr, k.shi};
+    const double hds = 0.5 * k.ds;
+
+    if (own) {
+#pragma unroll
+        for (int i = 0; i < C; ++i) d0[i] = inc ? inc[x0 + i] : cd{1.0, 0.0};
+        for (int z = 0; z < Nz; ++z)
+#pragma unroll
+            for (int i = 0; i < C; ++i) dg[(size_t)z * Nx + i] = cd{0.0, 0.0};
+    }
+    __syncthreads();
+    double res = __builtin_nan("");
+    int iters = n_iter, conv = 0, par = 0;
+    cd cur[C], cb[C], en[D][C], fn[D][C];
+    auto refresh = [&]() {
+        if (lane >= kWave - Hl) {
+#pragma unroll
+            for (int i = 0; i < C; ++i)
+                xch[par][wv][(lane - (kWave - Hl)) * C + i] = cur[i];
+        }
+        __syncthreads();
+        if (lane < Hl) {
+#pragma unroll
+            for (int i = 0; i < C; ++i) cur[i] = xch[par][wl][lane * C + i];
+        }
+        par ^= 1;
+    };
+    // forward step z with the set holding depth z + 1; reloads it with
+    // depth z + 1 + D
+    auto fwd = [&](int z, cd (&e)[C], cd (&f)[C]) {
+#pragma unroll
+        for (int i = 0; i < C; ++i) {
+            cur[i] = cmul(e0, cadd(cur[i], cb[i]));
+            cb[i] = cscale(cmul(cmul(sb, e[i]), f[i]), hds);
+        }
+        asm volatile("" ::: "memory");
+        const int zp = z + 1 + D < Nz ? z + 1 + D : Nz - 1;
+#pragma unroll
+        for (int i = 0; i < C; ++i) {
+            e[i] = Es[(size_t)zp * zs + i];
+            f[i] = dg[(size_t)zp * Nx + i];
+        }
+        const cd w = wave_fetch(cur[C - 1], src);
+#pragma unroll
+        for (int i = C - 1; i > 0; --i) cur[i] = cadd(cur[i - 1], cb[i]);
+        cur[0] = cadd(w, cb[0]);
+        cd *dst = own ? d0 + (size_t)(z + 1) * Nx : tr;
+#pragma unroll
+        for (int i = 0; i < C; ++i) dst[i] = cur[i];
+    };
+    // backward step z with the set holding depth z; reloads it with
+    // depth z - D
+    auto bwd = [&](int z, cd (&e)[C], cd (&f)[C]) {
+#pragma unroll
+        for (int i = 0; i < C; ++i) {
+            cur[i] = cmul(eg, cadd(cur[i], cb[i]));
+            cb[i] = cscale(cmul(cconjmul(e[i], sh), f[i]), hds);
+        }
+        asm volatile("" ::: "memory");
+        const int zp = z - D > 0 ? z - D : 0;
+#pragma unroll
+        for (int i = 0; i < C; ++i) {
+            e[i] = Es[(size_t)zp * zs + i];
+            f[i] = d0[(size_t)zp * Nx + i];
+        }
+        const cd w = wave_fetch(cur[C - 1], src);
+#pragma unroll
+        for (int i = C - 1; i > 0; --i) cur[i] = cadd(cur[i - 1], cb[i]);
+        cur[0] = cadd(w, cb[0]);
+        cd *dst = own && z >= 1 ? dg + (size_t)z * Nx : tr;
+#pragma unroll
+        for (int i = 0; i < C; ++i) dst[i] = cur[i];
+    };
+    const int ns = Nz - 1;                     // steps per sweep
+    for (int it = 0; it < n_iter; ++it) {
+        // forward: D0[z+1] = e0 roll(D0[z] + hds Cb[z] Dg[z]) + hds Cb[z+1] Dg[z+1]
+#pragma unroll
+        for (int i = 0; i < C; ++i) {
+            cur[i] = d0[i];
+            cb[i] = cscale(cmul(cmul(sb, Es[i]), dg[i]), hds);
+        }
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int zq = 1 + d < Nz ? 1 + d : Nz - 1;
+#pragma unroll
+            for (int i = 0; i < C; ++i) {
+                en[d][i] = Es[(size_t)zq * zs + i];
+                fn[d][i] = dg[(size_t)zq * Nx + i];
+            }
+        }
+        // the first group is peeled: the loop is then entered with its
+        // loads and stores in flight in the order of its back-edge, and
+        // each set waits only for its own loads (stores count too)
+        int t = 0, since = 0;
+        if (D <= ns) {
+#pragma unroll
+            for (int d = 0; d < D; ++d) fwd(d, en[d], fn[d]);
+            t = D, since = D;
+        }
+        for (; t + D <= ns; t += D) {
+            if (since == P) refresh(), since = 0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) fwd(t + d, en[d], fn[d]);
+            since += D;
+        }
+        if (t < ns && since == P) refresh();
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+            if (t + d < ns) fwd(t + d, en[d], fn[d]);
+        __syncthreads();                       // D0 of every column stored
+        // backward: Dg[z] = eg roll(Dg[z+1] + hds Ch[z+1] D0[z+1]) + hds Ch[z] D0[z]
+        if (own) {
+#pragma unroll
+            for (int i = 0; i < C; ++i)
+                dg[(size_t)(Nz - 1) * Nx + i] = cd{0.0, 0.0};
+        }
+#pragma unroll
+        for (int i = 0; i < C; ++i) {
+            const cd dz = d0[(size_t)(Nz - 1) * Nx + i];
+            cb[i] = cscale(cmul(cconjmul(Es[(size_t)(Nz - 1) * zs + i], sh), dz),
+                           hds);
+            cur[i] = cd{0.0, 0.0};
+        }
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int zq = Nz - 2 - d > 0 ? Nz - 2 - d : 0;
+#pragma unroll
+            for (int i = 0; i < C; ++i) {
+                en[d][i] = Es[(size_t)zq * zs + i];
+                fn[d][i] = d0[(size_t)zq * Nx + i];
+            }
+        }
+        t = 0, since = 0;
+        if (D <= ns) {
+#pragma unroll
+            for (int d = 0; d < D; ++d) bwd(Nz - 2 - d, en[d], fn[d]);
+            t = D, since = D;
+        }
+        for (; t + D <= ns; t += D) {
+            if (since == P) refresh(), since = 0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) bwd(Nz - 2 - (t + d), en[d], fn[d]);
+            since += D;
+        }
+        if (t < ns && since == P) refresh();
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+            if (t + d < ns) bwd(Nz - 2 - (t + d), en[d], fn[d]);
+        // convergence on the surface value Dg[0] against the last sweep's
+        double dmax = 0.0, amax = 0.0;
+        if (own) {
+#pragma unroll
+            for (int i = 0; i < C; ++i) {
+                const cd pv = dg[i];
+                const double dr = cur[i].r - pv.r, di = cur[i].i - pv.i;
+                dmax = fmax(dmax, sqrt(dr * dr + di * di));
+                amax = fmax(amax,
+                            sqrt(cur[i].r * cur[i].r + cur[i].i * cur[i].i));
+                dg[i] = cur[i];
+            }
+        }
+        dmax = wave_max(dmax);
+        amax = wave_max(amax);
+        if (lane == 0) red[0][wv] = dmax, red[1][wv] = amax;
+        __syncthreads();                       // also: Dg of every column stored
+#pragma unroll
+        for (int w = 0; w < W; ++w)
+            dmax = fmax(dmax, red[0][w]), amax = fmax(amax, red[1][w]);
+        __syncthreads();                       // red[] read before its reuse
+        if (it > 0) {
+            res = dmax / (amax + 1e-300);
+            if (res < tol) {
+                conv = 1, iters = it + 1;
+                break;
+            }
+        }
+    }
+    if (own) {
+#pragma unroll
+        for (int i = 0; i < C; ++i)
+            out[(size_t)blockIdx.x * Nx + x0 + i] = cconjmul(Es[i], cur[i]);
+    }
+    if (threadIdx.x == 0) info[blockIdx.x] = dyf_fixed_info{res, iters, conv, };
+}
+
+// Rows per launch below which the fixed-point solve splits rows over
+// wavefronts: at most this many wavefronts in all.
+constexpr long long kFpHaloWaves = 2048;
+
+// Wavefronts per row and columns per lane of the fixed-point halo kernel:
+// the largest W in {16, 8, 4, 2} with W | Nx and rows W <= kFpHaloWaves, and
+// the smallest C <= 8 (<= 4 at W = 16) with C | Nx/W, Lo = Nx/(W C) <= 63 owned lanes,
+// Hl = 64 - Lo <= Lo halo lanes and a halo of at least D(C) columns;
+// {0, 0} if none.
+static void fixed_halo_shape(int Nx, long long rows, int *W_, int *C_) {
+    *W_ = 0, *C_ = 0;
+    for (int W = 16; W >= 2; W /= 2) {
+        if (Nx % W || rows * W > kFpHaloWaves) continue;
+        const int Nw = Nx / W;
+        for (int c = 1; c <= (W == 16 ? 4 : 8); ++c) {   // as instantiated
+            if (Nw % c) continue;
+            const int Lo = Nw / c, Hl = kWave - Lo;
+            const int D = 8 / c > 2 ? 8 / c : 2;
+            if (Lo <= kWave - 1 && Hl <= Lo && Hl * c >= D) {
+                *W_ = W, *C_ = c;
+                return;
+            }
+        }
+    }
+}
+
+template <int W, int... Cs>
+static auto fixed_halo_fn(int C) {
+    decltype(&bragg_fixed_point_halo_kernel<1, 2>) f = nullptr;
+    ((C == Cs ? (f = bragg_fixed_point_halo_kernel<Cs, W>, 0) : 0), ...);
+    return f;
+}
+
 template <int... Cs>
 static auto fixed_wave_fn(int C) {
     decltype(&bragg_fixed_point_wave_kernel<1>) f = nullptr;
@@ -1566,8 +1831,11 @@ extern "C" int dyf_bragg_fixed_point_bytes(const dyf_bragg_args *a, int nsys,
     if (rc) return rc;
     if (!bytes) return fail(DYF_EINVAL, "bytes is NULL");
     const size_t n = (size_t)nsys * a->nyc;
+    int Wh = 0, Ch = 0;
+    fixed_halo_shape(a->Nx, (long long)n, &Wh, &Ch);
     *bytes = 2 * n * a->Nz * fixed_row(a->Nx) * sizeof(cd) +
-             n * sizeof(dyf_fixed_info);
+             n * sizeof(dyf_fixed_info) +
+             (Wh ? 256 + n * Wh * (kWave / 2) * Ch * sizeof(cd) : 0);
     return DYF_OK;
 }
 
@@ -1595,7 +1863,22 @@ extern "C" int dyf_bragg_fixed_point(const dyf_bragg_args *a,
     cd *D0 = (cd *)work, *Dg = D0 + field;
     dyf_fixed_info *idev = (dyf_fixed_info *)(Dg + field);
     const int C = wave_columns(a->Nx);
-    if (C) {
+    int Wh = 0, Ch = 0;
+    fixed_halo_shape(a->Nx, (long long)n, &Wh, &Ch);
+    if (Wh) {
+        auto hf = Wh == 16  ? fixed_halo_fn<16, 1, 2, 3, 4>(Ch)
+                  : Wh == 8 ? fixed_halo_fn<8, 1, 2, 3, 4, 5, 6, 7, 8>(Ch)
+                  : Wh == 4 ? fixed_halo_fn<4, 1, 2, 3, 4, 5, 6, 7, 8>(Ch)
+                            : fixed_halo_fn<2, 1, 2, 3, 4, 5, 6, 7, 8>(Ch);
+        if (!hf) return fail(DYF_EINVAL, "no fixed-point kernel for W=%d C=%d",
+                             Wh, Ch);
+        cd *trash = (cd *)(((uintptr_t)(idev + n) + 255) & ~(uintptr_t)255);
+        hipLaunchKernelGGL(hf, dim3((unsigned)n), dim3(kWave * Wh), 0, stream,
+                           k, s_dev, (const cd *)E, (const cd *)incident,
+                           opt->n_iter, opt->tol, D0, Dg, (cd *)out, idev,
+                           trash);
+        DYF_LAUNCHED("bragg_fixed_point_halo_kernel");
+    } else if (C) {
         auto wf = fixed_wave_fn<1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14,
                                 15, 16>(C);
         hipLaunchKernelGGL(wf, dim3((unsigned)n), dim3(kWave), 0, stream, k,

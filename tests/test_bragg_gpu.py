@@ -280,8 +280,29 @@ def test_20um_surface_with_edge_line_matches_oracle(dev, diamond400, g):
 
 # Nx = 432 = 54 lanes x 8 columns takes the wavefront fixed-point kernel,
 # Nx = 452 = 4 x 113 the workgroup-per-plane one.
+# Nx = 432 splits each row over 8 wavefronts (54 lanes x 1 column each);
+# Nx = 452 = 4 x 113 has no column count <= 16 for either wavefront form
+# and takes the workgroup kernel.
 FP_GRIDS = pytest.mark.parametrize("half_x", [2e-6, 3e-6],
-                                   ids=["wavefront-Nx432", "workgroup-Nx452"])
+                                   ids=["wavefronts-Nx432", "workgroup-Nx452"])
+
+
+def test_fixed_point_one_and_several_wavefronts_per_row_agree(dev,
+                                                              diamond400, g):
+    # 1100 rows take one wavefront per row (Nx 432 = 54 lanes x 8
+    # columns); one angle alone splits its row over 8 wavefronts
+    cfg = BraggConfig(diamond400, t_crystal=4e-6, dx=0.1e-6, half_x=2e-6)
+    seg = threading_segment("Edge", +1, diamond400.b_mag, float(g[2]),
+                            x0=0.3e-6, y0=0.05e-6)
+    s = deviation_from_urad(np.linspace(-30.0, 30.0, 1100), diamond400)
+    ews, infos = BS.solve_bragg(seg, s, cfg, diamond400, g, n_iter=800,
+                                tol=1e-13)
+    for j in (0, 377, 1099):
+        one, oi = BS.solve_bragg(seg, float(s[j]), cfg, diamond400, g,
+                                 n_iter=800, tol=1e-13)
+        assert oi["converged"] and infos[j]["converged"]
+        assert abs(oi["iterations"] - infos[j]["iterations"]) <= 1
+        assert _rel(one, ews[j]) < 1e-12
 
 
 @FP_GRIDS
