@@ -851,11 +851,11 @@ __global__ void arnoldi_finish_kernel(int nsys, int restart, int col, cd *H,
 // then ||w||^2), the Hessenberg/Givens update of arnoldi_finish_sys on
 // one lane, and v_{col+1} = w / ||w|| for a system that continues (what
 // the dot, combine, finish and scale kernels do in nine launches).
-// Reductions: a butterfly within each wavefront, then the four wavefront
-// sums in a fixed order.
-constexpr int kCgsThreads = 256;
-constexpr int kCgsPer = 8;
-constexpr int kCgsMaxN = kCgsThreads * kCgsPer;
+// Reductions: a butterfly within each wavefront, then the wavefront sums
+// in a fixed order.  The workgroup has NW = min(16, ceil(N / 64))
+// wavefronts, so most lanes hold one element.
+constexpr int kCgsMaxThreads = 1024;
+constexpr int kCgsMaxN = 2048;
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -863,20 +863,20 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// P = elements of w per lane (N <= 256 P).  Every lane loads at a
-// clamped in-range index, so the loads of a basis vector carry no branch
-// and two vectors' loads are in flight together; lanes past N hold w = 0
-// and keep it.
+// P = elements of w per lane (N <= P x block size).  Every lane loads at
+// a clamped in-range index, so the loads of a basis vector carry no
+// branch and two vectors' loads are in flight together; lanes past N hold
+// w = 0 and keep it.
 template <int P>
-__global__ __launch_bounds__(kCgsThreads) void arnoldi_step_kernel(
+__global__ __launch_bounds__(kCgsMaxThreads) void arnoldi_step_kernel(
     cd *__restrict__ V, size_t v_sys, size_t N, int restart, int col,
     const cd *__restrict__ w, cd *__restrict__ h1, cd *__restrict__ h2,
     cd *H, cd *G, cd *S, SysState *st, double *inv, int *active, double eps,
     int *still) {
-    extern __shared__ double red[];   // [4 waves][2 kc], h [2 kc]
-    const int s = blockIdx.x, tid = threadIdx.x;
+    extern __shared__ double red[];   // [NW waves][2 kc], h [2 kc]
+    const int s = blockIdx.x, tid = threadIdx.x, T = blockDim.x;
     if (!active[s]) return;
-    const int wv = tid >> 6, lane = tid & 63;
+    const int wv = tid >> 6, lane = tid & 63, NW = T >> 6;
     const int kc = col + 1, hstride = restart + 1;
     const cd *ws = w + (size_t)s * N;
     cd *Vs = V + (size_t)s * v_sys;
@@ -886,20 +886,22 @@ __global__ __launch_bounds__(kCgsThreads) void arnoldi_step_kernel(
     double nn = 0.0;
 #pragma unroll
     for (int m = 0; m < P; ++m) {
-        const size_t i = tid + (size_t)m * kCgsThreads;
+        const size_t i = tid + (size_t)m * T;
         ok[m] = i < N;
         idx[m] = ok[m] ? i : N - 1;
         const cd v = ws[idx[m]];
         x[m] = ok[m] ? v : cd{0.0, 0.0};
         nn += x[m].r * x[m].r + x[m].i * x[m].i;
     }
-    // block sum of one value through red[0..3]
+    // block sum of one value through red[0..NW)
     auto block_sum = [&](double v) {
         v = wave_sum(v);
         __syncthreads();
         if (lane == 0) red[wv] = v;
         __syncthreads();
-        return ((red[0] + red[1]) + red[2]) + red[3];
+        double t = red[0];
+        for (int u = 1; u < NW; ++u) t += red[u];
+        return t;
     };
     const double n0 = block_sum(nn);
     for (int pass = 0; pass < 2; ++pass) {
@@ -930,14 +932,13 @@ __global__ __launch_bounds__(kCgsThreads) void arnoldi_step_kernel(
             }
         }
         __syncthreads();
+        const int hb = 2 * NW * kc;           // h after the wave sums
         if (tid < kc) {
-            const double r = ((red[2 * tid] + red[2 * kc + 2 * tid]) +
-                              red[4 * kc + 2 * tid]) + red[6 * kc + 2 * tid];
-            const double i = ((red[2 * tid + 1] + red[2 * kc + 2 * tid + 1]) +
-                              red[4 * kc + 2 * tid + 1]) +
-                             red[6 * kc + 2 * tid + 1];
+            double r = red[2 * tid], i = red[2 * tid + 1];
+            for (int u = 1; u < NW; ++u)
+                r += red[u * 2 * kc + 2 * tid], i += red[u * 2 * kc + 2 * tid + 1];
             hp[tid] = cd{r, i};
-            red[8 * kc + 2 * tid] = r, red[8 * kc + 2 * tid + 1] = i;
+            red[hb + 2 * tid] = r, red[hb + 2 * tid + 1] = i;
         }
         __syncthreads();
         double ar[P], ai[P];
@@ -945,9 +946,9 @@ __global__ __launch_bounds__(kCgsThreads) void arnoldi_step_kernel(
         for (int m = 0; m < P; ++m) ar[m] = 0.0, ai[m] = 0.0;
         for (int q = 0; q < kc; q += 2) {
             const bool two = q + 1 < kc;
-            const double c0r = red[8 * kc + 2 * q], c0i = red[8 * kc + 2 * q + 1];
-            const double c1r = two ? red[8 * kc + 2 * q + 2] : 0.0;
-            const double c1i = two ? red[8 * kc + 2 * q + 3] : 0.0;
+            const double c0r = red[hb + 2 * q], c0i = red[hb + 2 * q + 1];
+            const double c1r = two ? red[hb + 2 * q + 2] : 0.0;
+            const double c1i = two ? red[hb + 2 * q + 3] : 0.0;
             const cd *v0 = Vs + (size_t)q * N;
             const cd *v1 = Vs + (size_t)(two ? q + 1 : q) * N;
             cd a[P], b[P];
@@ -972,14 +973,14 @@ __global__ __launch_bounds__(kCgsThreads) void arnoldi_step_kernel(
     for (int m = 0; m < P; ++m)
         nn += x[m].r * x[m].r + x[m].i * x[m].i;
     nn = block_sum(nn);
-    // red[0..3] may still be read by other lanes: broadcast through red[4]
+    // red[0..NW) may still be read by other lanes: broadcast through red[NW]
     if (tid == 0)
-        red[4] = arnoldi_finish_sys(s, restart, col, H, G, S, h1, h2, nn, n0,
-                                    st, inv, active, eps, still)
-                     ? inv[s]
-                     : 0.0;
+        red[NW] = arnoldi_finish_sys(s, restart, col, H, G, S, h1, h2, nn, n0,
+                                     st, inv, active, eps, still)
+                      ? inv[s]
+                      : 0.0;
     __syncthreads();
-    const double sc = red[4];
+    const double sc = red[NW];
     if (sc == 0.0) return;
     cd *vn = Vs + (size_t)kc * N;
 #pragma unroll
@@ -991,12 +992,22 @@ using step_fn = void (*)(cd *, size_t, size_t, int, int, const cd *, cd *,
                          cd *, cd *, cd *, cd *, SysState *, double *, int *,
                          double, int *);
 
+// Workgroup size of the Arnoldi step for N unknowns (N <= kCgsMaxN):
+// about kCgsPer elements per lane, at most 8
+constexpr int kCgsPer = 4;
+static int step_threads(size_t N) {
+    const size_t waves = std::max<size_t>(
+        (N + 64 * kCgsPer - 1) / (64 * kCgsPer), (N + 511) / 512);
+    return (int)std::min<size_t>(kCgsMaxThreads, waves * 64);
+}
+
 static step_fn step_kernel(size_t N) {
-    static const step_fn tab[kCgsPer] = {
+    static const step_fn tab[8] = {
         arnoldi_step_kernel<1>, arnoldi_step_kernel<2>, arnoldi_step_kernel<3>,
         arnoldi_step_kernel<4>, arnoldi_step_kernel<5>, arnoldi_step_kernel<6>,
         arnoldi_step_kernel<7>, arnoldi_step_kernel<8>};
-    return tab[(N + kCgsThreads - 1) / kCgsThreads - 1];
+    const size_t T = step_threads(N);
+    return tab[(N + T - 1) / T - 1];
 }
 
 // Start of a restart cycle: inv_s = 1/||M r_s||, S_s = [||M r_s||, 0, ...]
@@ -2016,9 +2027,11 @@ extern "C" int dyf_bragg_solve(const dyf_bragg_args *a, const double *s_dev,
             rc = apply_precon(k, W.pinv, W.av, W.w, stream);
             if (rc) return rc;
             // h0 = ||w||; CGS2 against v_0..v_col
-            if (N <= (size_t)kCgsMaxN) {
-                const size_t lds = sizeof(double) * 10 * (size_t)(col + 1);
-                step_kernel(N)<<<nsys, kCgsThreads, lds, stream>>>(
+            if (N <= (size_t)kCgsMaxN && R < step_threads(N)) {
+                const int T = step_threads(N);
+                const size_t lds =
+                    sizeof(double) * (2 * (T / 64) + 2) * (size_t)(col + 1);
+                step_kernel(N)<<<nsys, T, lds, stream>>>(
                     W.V, Vsys, N, R, col, W.w, W.h1, W.h2, W.H, W.G, W.S,
                     W.st, W.inv, W.active, eps, W.still + col);
                 DYF_LAUNCHED("arnoldi_step_kernel");
