@@ -1508,7 +1508,7 @@ __global__ __launch_bounds__(kWave) void bragg_fixed_point_wave_kernel(
 // read columns other wavefronts of the workgroup stored; the barrier
 // between sweeps orders those global stores before the loads (the
 // wavefronts of a workgroup share one CU and its vector L1).
-template <int C, int W, int D = (8 / C > 2 ? 8 / C : 2)>
+template <int C, int W, int D = (4 / C > 2 ? 4 / C : 2)>
 __global__ __launch_bounds__(kWave * W) void bragg_fixed_point_halo_kernel(
     BraggK k, const double *__restrict__ s_dev, const cd *__restrict__ E,
     const cd *__restrict__ inc, int n_iter, double tol, cd *__restrict__ D0,
@@ -1737,7 +1737,7 @@ static void fixed_halo_shape(int Nx, long long rows, int *W_, int *C_) {
         for (int c = 1; c <= (W == 16 ? 4 : 8); ++c) {   // as instantiated
             if (Nw % c) continue;
             const int Lo = Nw / c, Hl = kWave - Lo;
-            const int D = 8 / c > 2 ? 8 / c : 2;
+            const int D = 4 / c > 2 ? 4 / c : 2;
             if (Lo <= kWave - 1 && Hl <= Lo && Hl * c >= D) {
                 *W_ = W, *C_ = c;
                 return;
