@@ -226,10 +226,16 @@ __device__ __forceinline__ void fast_row(const FastArgs &a, int r, int ie) {
     // as column i of step k has used it, so every load has a whole step
     // to land and no second slice of registers is held (C = 17 FP64: 68
     // VGPRs fewer, a third wave per SIMD).
+    // In FP32 the first slice is loaded in column order, as the loop
+    // reloads it: the wait counts at the loop head are then exact
+    // (vmcnt(C - 1)) instead of a full drain.
     cx<T> Ec[MODE == DYF_MODE_GRID ? C : 1];
     if constexpr (MODE == DYF_MODE_GRID) {
 #pragma unroll
-        for (int i = 0; i < C; ++i) Ec[i] = E[i * kWave];
+        for (int i = 0; i < C; ++i) {
+            Ec[i] = E[i * kWave];
+            if constexpr (sizeof(T) == 4) __builtin_amdgcn_sched_barrier(0);
+        }
     }
 
     for (int k = 0; k < a.nsteps; ++k) {
@@ -279,9 +285,13 @@ __device__ __forceinline__ void fast_row(const FastArgs &a, int r, int ie) {
                 ur = fmaT(er, xr, ei * xi);
                 ui = fmaT(er, xi, -ei * xr);
                 // the next step's value, into the register just read (the
-                // fence keeps the load from being hoisted into a copy)
+                // fence keeps the load from being hoisted into a copy).  In
+                // FP32 a scheduling barrier keeps it from sinking to the end
+                // of the step, so it has a whole step to land (mixed march
+                // -2 %; in FP64 the barrier cost 1 %)
                 asm volatile("" ::: "memory");
                 Ec[i] = Ek[i * kWave];
+                if constexpr (sizeof(T) == 4) __builtin_amdgcn_sched_barrier(0);
             } else {
                 tr = msk[i] * yr, ti = msk[i] * yi;
                 ur = msk[i] * xr, ui = msk[i] * xi;
