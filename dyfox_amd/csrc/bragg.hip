@@ -2007,6 +2007,10 @@ extern "C" int dyf_bragg_solve(const dyf_bragg_args *a, const double *s_dev,
     // look at the set-up: if no system needs it, it is masked out the
     // same way and its end-of-cycle word ends the loop.
     const unsigned sgrid = (unsigned)(((size_t)nsys * (R + 1) + 255) / 256);
+    const bool fused_step =
+        N <= (size_t)kCgsMaxN && R < step_threads(N) &&
+        sizeof(double) * (2 * (step_threads(N) / 64) + 2) * (size_t)R <=
+            64 * 1024;
 
     for (int cycle = 0; cycle < opt->maxiter; ++cycle) {
         // v0 = M r / ||M r||, S = [||M r||, 0, ...]
@@ -2027,7 +2031,9 @@ extern "C" int dyf_bragg_solve(const dyf_bragg_args *a, const double *s_dev,
             rc = apply_precon(k, W.pinv, W.av, W.w, stream);
             if (rc) return rc;
             // h0 = ||w||; CGS2 against v_0..v_col
-            if (N <= (size_t)kCgsMaxN && R < step_threads(N)) {
+            // one workgroup per system while h fits its lanes and its
+            // wave sums fit the default 64 KB of LDS at the last column
+            if (fused_step) {
                 const int T = step_threads(N);
                 const size_t lds =
                     sizeof(double) * (2 * (T / 64) + 2) * (size_t)(col + 1);
