@@ -551,13 +551,27 @@ __global__ __launch_bounds__(256) void bragg_symbol_kernel(
         const cd a12 = cmul(cscale(cmul(e0, p), k.ds), sb);
         const cd a21 = cdiv(cmul(cscale(cd{-pc.r, -pc.i}, k.ds), sh), eg);
         const cd a22 = cdiv(pc, eg);
+        // T = A^(Nz-1) by repeated squaring (the step matrix A is the
+        // same at every depth): about 2 log2(Nz) 2x2 products instead of
+        // Nz - 1 in a dependent chain; the rounding differs from the
+        // step-by-step product in the last bits only
         cd T11 = {1, 0}, T12 = {0, 0}, T21 = {0, 0}, T22 = {1, 0};
-        for (int st = 0; st < k.Nz - 1; ++st) {
-            const cd n11 = cadd(cmul(a11, T11), cmul(a12, T21));
-            const cd n12 = cadd(cmul(a11, T12), cmul(a12, T22));
-            const cd n21 = cadd(cmul(a21, T11), cmul(a22, T21));
-            const cd n22 = cadd(cmul(a21, T12), cmul(a22, T22));
-            T11 = n11, T12 = n12, T21 = n21, T22 = n22;
+        cd p11 = a11, p12 = a12, p21 = a21, p22 = a22;
+        for (int e = k.Nz - 1; e > 0; e >>= 1) {
+            if (e & 1) {
+                const cd n11 = cadd(cmul(p11, T11), cmul(p12, T21));
+                const cd n12 = cadd(cmul(p11, T12), cmul(p12, T22));
+                const cd n21 = cadd(cmul(p21, T11), cmul(p22, T21));
+                const cd n22 = cadd(cmul(p21, T12), cmul(p22, T22));
+                T11 = n11, T12 = n12, T21 = n21, T22 = n22;
+            }
+            if (e > 1) {
+                const cd q11 = cadd(cmul(p11, p11), cmul(p12, p21));
+                const cd q12 = cadd(cmul(p11, p12), cmul(p12, p22));
+                const cd q21 = cadd(cmul(p21, p11), cmul(p22, p21));
+                const cd q22 = cadd(cmul(p21, p12), cmul(p22, p22));
+                p11 = q11, p12 = q12, p21 = q21, p22 = q22;
+            }
         }
         T[idx] = T22;
         mx = fmax(mx, sqrt(T22.r * T22.r + T22.i * T22.i));
