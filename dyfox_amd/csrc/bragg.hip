@@ -871,10 +871,33 @@ __global__ void arnoldi_finish_kernel(int nsys, int restart, int col, cd *H,
 constexpr int kCgsMaxThreads = 1024;
 constexpr int kCgsMaxN = 2048;
 
+// Cross-lane moves of a double through DPP (two 32-bit halves), and one
+// lane's double read into a uniform value.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi =
+        __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double lane_f64(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Sum over the 64 lanes of a fully active wavefront, in every lane: DPP
+// butterflies within each row of 16 lanes (swap pairs, swap pair-pairs,
+// rotate by 4, rotate by 8), then the four row sums in a fixed order.
+// (ds_bpermute butterflies cost an LDS round trip per stage.)
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    v += dpp_f64<0xB1>(v);    // quad_perm [1, 0, 3, 2]
+    v += dpp_f64<0x4E>(v);    // quad_perm [2, 3, 0, 1]
+    v += dpp_f64<0x124>(v);   // row_ror:4
+    v += dpp_f64<0x128>(v);   // row_ror:8
+    return (lane_f64(v, 0) + lane_f64(v, 16)) + (lane_f64(v, 32) + lane_f64(v, 48));
 }
 
 // P = elements of w per lane (N <= P x block size).  Every lane loads at
