@@ -647,7 +647,7 @@ static int apply_precon(const BraggK &k, const cd *pinv, cd *v, cd *w,
 
 // ---- batched reductions ----------------------------------------------
 // part[(s*kc + q)*P + p] = sum over chunk p of conj(A_sq[i]) * B_s[i]
-constexpr int kRedThreads = 256;
+constexpr int kRedThreads = 1024;
 
 __global__ __launch_bounds__(kRedThreads) void batched_dot_kernel(
     const cd *__restrict__ A, size_t a_sys, size_t a_vec, int kc,
@@ -663,6 +663,7 @@ __global__ __launch_bounds__(kRedThreads) void batched_dot_kernel(
     const size_t chunk = (N + P - 1) / P;
     const size_t lo = (size_t)p * chunk, hi = std::min(N, lo + chunk);
     double sr = 0.0, si = 0.0;
+#pragma unroll 4
     for (size_t i = lo + threadIdx.x; i < hi; i += kRedThreads) {
         const cd x = a[i], y = b[i];
         sr += x.r * y.r + x.i * y.i;
@@ -678,7 +679,7 @@ __global__ __launch_bounds__(kRedThreads) void batched_dot_kernel(
         }
         __syncthreads();
     }
-    // one part (N <= 4096): the block's sum is the dot product itself
+    // one part (N <= 32768): the block's sum is the dot product itself
     if (threadIdx.x == 0) {
         if (P == 1)
             out[(size_t)s * ostride + q] = cd{rr[0], ri[0]};
@@ -1175,8 +1176,10 @@ struct Work {
 
 static size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
+// parts of a batched dot product: one 1024-lane workgroup per 32768
+// elements (four loads of each vector in flight per lane)
 static int reduce_parts(size_t N) {
-    const int P = (int)std::min<size_t>((N + 4095) / 4096, 64);
+    const int P = (int)std::min<size_t>((N + 32767) / 32768, 64);
     return std::max(P, 1);
 }
 
