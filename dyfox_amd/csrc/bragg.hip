@@ -579,7 +579,8 @@ __global__ __launch_bounds__(256) void bragg_symbol_kernel(
     red[threadIdx.x] = mx;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        if ((int)threadIdx.x < w)
+            red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
         __syncthreads();
     }
     const double floor_v = 1e-8 * red[0];
@@ -673,7 +674,7 @@ __global__ __launch_bounds__(kRedThreads) void batched_dot_kernel(
     ri[threadIdx.x] = si;
     __syncthreads();
     for (int w = kRedThreads / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) {
+        if ((int)threadIdx.x < w) {
             rr[threadIdx.x] += rr[threadIdx.x + w];
             ri[threadIdx.x] += ri[threadIdx.x + w];
         }
@@ -1272,7 +1273,7 @@ __global__ __launch_bounds__(kFpThreads) void bragg_fixed_point_kernel(
     const cd sb = {k.sbr, k.sbi}, sh = {k.shr, k.shi};
     const double hds = 0.5 * k.ds;
 
-    cd a[kMaxPerThread], b[kMaxPerThread], pv[kMaxPerThread];
+    cd pv[kMaxPerThread];
 #pragma unroll
     for (int m = 0; m < kMaxPerThread; ++m) {
         const int x = tid + m * kFpThreads;
