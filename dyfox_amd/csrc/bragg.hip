@@ -790,11 +790,16 @@ __device__ void lartg(cd f, cd g, double *c, cd *s, cd *r) {
 // which clears the system's active flag; `still` is set while any system
 // continues, so the host reads one int per column.  nw = ||w||^2 after
 // CGS2, n0 = ||w||^2 before it.  Returns whether system s continues.
+// h1c, h2c: this column's projections from the two CGS2 passes (r <=
+// col); gp: the stored rotations (2 col entries); hc: a working column of
+// col + 2 entries, H's own column or a faster copy written to it at the
+// end; Sc = S[col].  (The fused step passes LDS copies, so the serial
+// rotation loop does not wait on global memory at every entry.)
 __device__ int arnoldi_finish_sys(int s, int restart, int col, cd *H, cd *G,
-                                  cd *S, const cd *h1st, const cd *h2nd,
-                                  double nw, double n0, SysState *st,
-                                  double *inv, int *active, double eps,
-                                  int *still) {
+                                  cd *S, const cd *h1c, const cd *h2c,
+                                  const cd *gp, cd *hc, cd Sc, double nw,
+                                  double n0, SysState *st, double *inv,
+                                  int *active, double eps, int *still) {
     SysState &q = st[s];
     q.col = col;
     q.h0 = sqrt(n0);
@@ -802,38 +807,39 @@ __device__ int arnoldi_finish_sys(int s, int restart, int col, cd *H, cd *G,
     cd *g = G + (size_t)s * restart * 2;
     cd *Sv = S + (size_t)s * (restart + 1);
     for (int r = 0; r <= col; ++r) {
-        const cd a = h1st[(size_t)s * (restart + 1) + r];
-        const cd b = h2nd[(size_t)s * (restart + 1) + r];
-        h[r] = cd{a.r + b.r, a.i + b.i};
+        const cd a = h1c[r];
+        const cd b = h2c[r];
+        hc[r] = cd{a.r + b.r, a.i + b.i};
     }
     const double h1 = sqrt(nw);
-    h[col + 1] = cd{h1, 0.0};
+    hc[col + 1] = cd{h1, 0.0};
     int breakdown = 0;
     if (h1 <= eps * q.h0) {
-        h[col + 1] = cd{0.0, 0.0};
+        hc[col + 1] = cd{0.0, 0.0};
         breakdown = 1;
         inv[s] = 0.0;
     } else {
         inv[s] = 1.0 / h1;
     }
     for (int r = 0; r < col; ++r) {
-        const double c = g[2 * r].r;
-        const cd sn = g[2 * r + 1];
-        const cd n0 = h[r], n1 = h[r + 1];
-        h[r] = cd{c * n0.r + (sn.r * n1.r - sn.i * n1.i),
-                  c * n0.i + (sn.r * n1.i + sn.i * n1.r)};
+        const double c = gp[2 * r].r;
+        const cd sn = gp[2 * r + 1];
+        const cd n0 = hc[r], n1 = hc[r + 1];
+        hc[r] = cd{c * n0.r + (sn.r * n1.r - sn.i * n1.i),
+                   c * n0.i + (sn.r * n1.i + sn.i * n1.r)};
         // -conj(s) n0 + c n1
-        h[r + 1] = cd{-(sn.r * n0.r + sn.i * n0.i) + c * n1.r,
-                      -(sn.r * n0.i - sn.i * n0.r) + c * n1.i};
+        hc[r + 1] = cd{-(sn.r * n0.r + sn.i * n0.i) + c * n1.r,
+                       -(sn.r * n0.i - sn.i * n0.r) + c * n1.i};
     }
     double c;
     cd sn, mag;
-    lartg(h[col], h[col + 1], &c, &sn, &mag);
+    lartg(hc[col], hc[col + 1], &c, &sn, &mag);
     g[2 * col] = cd{c, 0.0};
     g[2 * col + 1] = sn;
-    h[col] = mag;
-    h[col + 1] = cd{0.0, 0.0};
-    const cd Sc = Sv[col];
+    hc[col] = mag;
+    hc[col + 1] = cd{0.0, 0.0};
+    if (hc != h)
+        for (int r = 0; r <= col + 1; ++r) h[r] = hc[r];
     const cd tmp = {-(sn.r * Sc.r + sn.i * Sc.i), -(sn.r * Sc.i - sn.i * Sc.r)};
     Sv[col] = cd{c * Sc.r, c * Sc.i};
     Sv[col + 1] = tmp;
@@ -857,7 +863,12 @@ __global__ void arnoldi_finish_kernel(int nsys, int restart, int col, cd *H,
                                       int *still) {
     const int s = blockIdx.x * 64 + threadIdx.x;
     if (s >= nsys || !active[s]) return;
-    arnoldi_finish_sys(s, restart, col, H, G, S, h1st, h2nd, nrm_w[s].r,
+    cd *h = H + (size_t)s * restart * (restart + 1) + (size_t)col * (restart + 1);
+    arnoldi_finish_sys(s, restart, col, H, G, S,
+                       h1st + (size_t)s * (restart + 1),
+                       h2nd + (size_t)s * (restart + 1),
+                       G + (size_t)s * restart * 2, h,
+                       S[(size_t)s * (restart + 1) + col], nrm_w[s].r,
                        nrm0[s].r, st, inv, active, eps, still);
 }
 
@@ -912,11 +923,20 @@ __global__ __launch_bounds__(kCgsMaxThreads) void arnoldi_step_kernel(
     const cd *__restrict__ w, cd *__restrict__ h1, cd *__restrict__ h2,
     cd *H, cd *G, cd *S, SysState *st, double *inv, int *active, double eps,
     int *still) {
-    extern __shared__ double red[];   // [NW waves][2 kc], h [2 kc]
+    // LDS (doubles): [NW waves][2 kc] wave sums, h [2 kc], then the first
+    // pass's h and the working column [2 (kc + 1)], then the stored
+    // rotations [4 col]
+    extern __shared__ double red[];
     const int s = blockIdx.x, tid = threadIdx.x, T = blockDim.x;
     if (!active[s]) return;
     const int wv = tid >> 6, lane = tid & 63, NW = T >> 6;
     const int kc = col + 1, hstride = restart + 1;
+    const int hb = 2 * NW * kc;               // h after the wave sums
+    cd *h1l = (cd *)(red + hb + 2 * kc);
+    cd *gl = h1l + (kc + 1);
+    for (int t = tid; t < 2 * col; t += T)
+        gl[t] = G[(size_t)s * restart * 2 + t];
+    const cd Sc0 = tid == 0 ? S[(size_t)s * hstride + col] : cd{0.0, 0.0};
     const cd *ws = w + (size_t)s * N;
     cd *Vs = V + (size_t)s * v_sys;
     cd x[P];
@@ -971,13 +991,13 @@ __global__ __launch_bounds__(kCgsMaxThreads) void arnoldi_step_kernel(
             }
         }
         __syncthreads();
-        const int hb = 2 * NW * kc;           // h after the wave sums
         if (tid < kc) {
             double r = red[2 * tid], i = red[2 * tid + 1];
             for (int u = 1; u < NW; ++u)
                 r += red[u * 2 * kc + 2 * tid], i += red[u * 2 * kc + 2 * tid + 1];
             hp[tid] = cd{r, i};
             red[hb + 2 * tid] = r, red[hb + 2 * tid + 1] = i;
+            if (pass == 0) h1l[tid] = cd{r, i};
         }
         __syncthreads();
         double ar[P], ai[P];
@@ -1014,8 +1034,9 @@ __global__ __launch_bounds__(kCgsMaxThreads) void arnoldi_step_kernel(
     nn = block_sum(nn);
     // red[0..NW) may still be read by other lanes: broadcast through red[NW]
     if (tid == 0)
-        red[NW] = arnoldi_finish_sys(s, restart, col, H, G, S, h1, h2, nn, n0,
-                                     st, inv, active, eps, still)
+        red[NW] = arnoldi_finish_sys(s, restart, col, H, G, S, h1l,
+                                     (const cd *)(red + hb), gl, h1l, Sc0,
+                                     nn, n0, st, inv, active, eps, still)
                       ? inv[s]
                       : 0.0;
     __syncthreads();
@@ -2050,7 +2071,7 @@ extern "C" int dyf_bragg_solve(const dyf_bragg_args *a, const double *s_dev,
     const unsigned sgrid = (unsigned)(((size_t)nsys * (R + 1) + 255) / 256);
     const bool fused_step =
         N <= (size_t)kCgsMaxN && R < step_threads(N) &&
-        sizeof(double) * (2 * (step_threads(N) / 64) + 2) * (size_t)R <=
+        sizeof(double) * (2 * (step_threads(N) / 64) + 8) * (size_t)R <=
             64 * 1024;
 
     for (int cycle = 0; cycle < opt->maxiter; ++cycle) {
@@ -2077,7 +2098,7 @@ extern "C" int dyf_bragg_solve(const dyf_bragg_args *a, const double *s_dev,
             if (fused_step) {
                 const int T = step_threads(N);
                 const size_t lds =
-                    sizeof(double) * (2 * (T / 64) + 2) * (size_t)(col + 1);
+                    sizeof(double) * (2 * (T / 64) + 8) * (size_t)(col + 1);
                 step_kernel(N)<<<nsys, T, lds, stream>>>(
                     W.V, Vsys, N, R, col, W.w, W.h1, W.h2, W.H, W.G, W.S,
                     W.st, W.inv, W.active, eps, W.still + col);
