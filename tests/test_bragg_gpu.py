@@ -128,6 +128,28 @@ def test_surface_batch_matches_oracle(dev, diamond400, g):
         assert np.abs(got[j] - one).max() < 1e-7 * np.abs(one).max()
 
 
+def test_fused_and_separate_arnoldi_steps_agree(dev, diamond400, g):
+    # Nx 324: plane systems of 324 unknowns take the one-workgroup Arnoldi
+    # step at restart 40 and the separate dot/combine/finish kernels at
+    # restart 200 (h would not fit the workgroup's lanes); with fewer
+    # iterations than either restart the two Krylov sequences are the same
+    cfg = BraggConfig(diamond400, t_crystal=6e-6, dx=0.2e-6, half_x=2e-6)
+    seg = threading_segment("Edge", +1, diamond400.b_mag, float(g[2]))
+    ys = np.array([-0.3e-6, 0.1e-6, 0.5e-6])
+    E = BS.chunk_phase(seg, cfg, diamond400, g, ys)
+    s = deviation_from_urad(np.array([-6.5, 12.0]), diamond400)
+    inc = cfg.incident(n=1)[0]
+    runs = [BS.gmres_batch(cfg, diamond400, s, E, inc, tol=1e-10, restart=r,
+                           maxiter=20, planes=True) for r in (40, 200)]
+    (x40, i40), (x200, i200) = runs
+    # same sequence up to the reductions' rounding: counts within one,
+    # solutions to the GMRES bar used against the oracle
+    assert i40["iterations"].max() < 40
+    assert np.abs(i40["iterations"] - i200["iterations"]).max() <= 1
+    assert (i40["code"] == 0).all() and (i200["code"] == 0).all()
+    assert _rel(x40.get(), x200.get()) < 1e-8
+
+
 def test_joint_chunks_follow_scipy_and_agree_with_planes(dev, diamond400,
                                                          g):
     """planes="joint" is the reference's chunk (SciPy's iteration count);
