@@ -309,6 +309,25 @@ FP_GRIDS = pytest.mark.parametrize("half_x", [2e-6, 3e-6],
                                    ids=["wavefronts-Nx432", "workgroup-Nx452"])
 
 
+def test_fixed_point_benchmark_grid_matches_oracle(dev, diamond400, g):
+    # the benchmark's grid (t = 8 um, Nz 180, Nx 832): 16 wavefronts per
+    # row, one column per lane
+    cfg = BraggConfig(diamond400, t_crystal=8e-6, dx=0.1e-6, half_x=4e-6)
+    assert (cfg.Nz, cfg.Nx) == (180, 832)
+    seg = threading_segment("Edge", +1, diamond400.b_mag, float(g[2]),
+                            x0=0.3e-6, y0=0.05e-6)
+    s = deviation_from_urad(np.array([-20.0, 0.0, 35.0]), diamond400)
+    ews, infos = BS.solve_bragg(seg, s, cfg, diamond400, g, n_iter=600,
+                                tol=1e-13)
+    for j in (0, 2):
+        want, (its, res, conv) = OB.fixed_point(seg, float(s[j]), cfg,
+                                                diamond400, g, n_iter=600,
+                                                tol=1e-13)
+        assert infos[j]["converged"] and conv
+        assert abs(infos[j]["iterations"] - its) <= 1
+        assert _rel(ews[j], want) < 1e-11
+
+
 def test_fixed_point_one_and_several_wavefronts_per_row_agree(dev,
                                                               diamond400, g):
     # 1100 rows take one wavefront per row (Nx 432 = 54 lanes x 8
