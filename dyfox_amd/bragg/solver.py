@@ -182,19 +182,24 @@ def solve_surface_batch(seg_list, s_dev, cfg, y_values, xtal, g_vec,
                         maxiter=25, verbose=False, gpu_gmres=True,
                         accept=1e-7, grow=4, tries=3, restart_cap=256,
                         max_depth=6, per_plane=6, chunk_cap=32,
-                        warm_start=None, planes="independent"):
+                        warm_start=None, planes="joint"):
     """Exit wave on the (ny, Nx) surface grid (see module docstring).
 
-    planes="joint" reproduces the reference's chunking: the nyc planes of
-    a chunk share one Krylov space of per_plane * nyc directions, and
-    chunks that resist are split.  planes="independent" (the default)
-    solves every plane of a chunk as its own system in the same batched
-    GMRES, sized by the reference's rule at one plane (restart 40): the
-    planes are independent problems (bragg/solver.py:189-191), so each
-    then converges on its own spectrum, 4x-30x fewer plane-marches in
-    total on strained crystals 20-38 um thick, to the same tolerance.
-    Each plane meets rtol on its own, which implies the chunk's joint
-    criterion.  "unconverged_chunks" then counts planes.
+    planes="joint" (the default) is the reference's chunking, with its
+    integer `info`: the nyc planes of a chunk share one Krylov space of
+    per_plane * nyc directions, and chunks that resist are split.
+    planes="independent" (opt-in) solves every plane of a chunk as its
+    own system in the same batched GMRES: the planes are independent
+    problems (bragg/solver.py:189-191), so each converges on its own
+    spectrum, 4x-30x fewer plane-marches in total on strained crystals
+    20-38 um thick, to the same tolerance (each plane meets rtol on its
+    own, which implies the chunk's joint criterion).  Each plane follows
+    the reference's rule for a one-plane chunk: Krylov size
+    max(restart, min(cap, per_plane)) with cap = max(restart,
+    min(restart_cap, 8)), so at the default restart 40 the size is
+    already the cap and no plane is retried (as in the reference's
+    one-plane block); nothing is split, and "unconverged_chunks" counts
+    planes.
 
     Returns (out, info): out (ny, Nx) complex128 NumPy for a scalar s_dev
     and info with the reference's keys ("residual", "iterations",

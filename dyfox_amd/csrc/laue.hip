@@ -817,6 +817,29 @@ extern "C" int dyf_laue_march(const dyf_laue_args *a, int mode, int prec,
     if (n_e < 0) return fail(DYF_EINVAL, "n_e < 0");
     if (n_e == 0) return DYF_OK;
     if (!out || !s_dev) return fail(DYF_EINVAL, "NULL output or s_dev");
+    // every argument is checked before the first launch, so an error
+    // leaves nothing enqueued (the reference raises before launching:
+    // laue/kernels.py:365-386)
+    const size_t lds = (size_t)2 * 4 * (a->Nx + 1) * sizeof(double);
+    if (L.path == DYF_PATH_FAST && L.nrows > 0) {
+        if (mode != DYF_MODE_PRISTINE && !src)
+            return fail(DYF_EINVAL, "phase table is NULL");
+        if (!coef) return fail(DYF_EINVAL, "coefficient workspace is NULL");
+        if (!pick_fast(mode, prec, L.C))
+            return fail(DYF_EKEY, "no fast kernel for C=%d", L.C);
+    } else if (L.nrows > 0) {
+        if (mode == DYF_MODE_GRID && !src)
+            return fail(DYF_EINVAL, "H grid is NULL");
+        if (mode == DYF_MODE_GRID && H_prec != DYF_FP64 && H_prec != DYF_FP32)
+            return fail(DYF_EKEY, "H precision must be DYF_FP64 or DYF_FP32");
+        if (mode == DYF_MODE_ANALYTIC && !line)
+            return fail(DYF_EINVAL, "line parameters are NULL");
+        if (lds > 160 * 1024)
+            return fail(DYF_ENOMEM,
+                        "the general march needs %zu KB of LDS per workgroup "
+                        "but the device allows 160 KB; reduce the x-grid",
+                        lds / 1024);
+    }
     const int elem = prec == DYF_FP64 ? 16 : 8;
     const int nrows = L.nrows;
     if (nrows < a->Ny) {
@@ -828,9 +851,6 @@ extern "C" int dyf_laue_march(const dyf_laue_args *a, int mode, int prec,
     }
     if (nrows == 0) return DYF_OK;
     if (L.path == DYF_PATH_FAST) {
-        if (mode != DYF_MODE_PRISTINE && !src)
-            return fail(DYF_EINVAL, "phase table is NULL");
-        if (!coef) return fail(DYF_EINVAL, "coefficient workspace is NULL");
         const CoefPtrs cp = coef_ptrs(coef, n_e, L.nsteps);
         const int ncoef = n_e * L.nsteps;
         laue_coef_kernel<<<(ncoef + 255) / 256, 256, 0, stream>>>(
@@ -854,23 +874,11 @@ extern "C" int dyf_laue_march(const dyf_laue_args *a, int mode, int prec,
         fa.dx = a->dx;
         fa.sheet_hw = a->sheet_hw;
         FastFn fn = pick_fast(mode, prec, L.C);
-        if (!fn) return fail(DYF_EKEY, "no fast kernel for C=%d", L.C);
         hipLaunchKernelGGL(fn, dim3(8 * fa.per_xcd), dim3(kWave), 0, stream, fa);
         DYF_LAUNCHED("laue_fast_kernel");
         return DYF_OK;
     }
     // general path
-    if (mode == DYF_MODE_GRID && !src) return fail(DYF_EINVAL, "H grid is NULL");
-    if (mode == DYF_MODE_GRID && H_prec != DYF_FP64 && H_prec != DYF_FP32)
-        return fail(DYF_EKEY, "H precision must be DYF_FP64 or DYF_FP32");
-    if (mode == DYF_MODE_ANALYTIC && !line)
-        return fail(DYF_EINVAL, "line parameters are NULL");
-    const size_t lds = (size_t)2 * 4 * (a->Nx + 1) * sizeof(double);
-    if (lds > 160 * 1024)
-        return fail(DYF_ENOMEM,
-                    "the general march needs %zu KB of LDS per workgroup but "
-                    "the device allows 160 KB; reduce the x-grid",
-                    lds / 1024);
     GeneralArgs ga;
     ga.H = src;
     ga.H_f32 = H_prec == DYF_FP32;

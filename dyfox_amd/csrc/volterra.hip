@@ -88,7 +88,9 @@ __device__ __forceinline__ void phase_table_points(
         return;
     }
     const double xp = ((double)x - (double)Nx / 2.0) * dx;
-    const double yp = (double)(iy_lo + r - Ny / 2) * dx;
+    // y as compute_H_grid places it (dislocations.py:146): (iy - Ny/2) dx
+    // with a real Ny/2, so an odd Ny puts rows half a pixel off the integer
+    const double yp = ((double)(iy_lo + r) - (double)Ny * 0.5) * dx;
     double z[kPhaseZ], Pr[kPhaseZ], Pi[kPhaseZ], R[kPhaseZ];
 #pragma unroll
     for (int j = 0; j < kPhaseZ; ++j) z[j] = (double)(k0 + j) * dz;
@@ -150,6 +152,9 @@ __global__ __launch_bounds__(256) void phase_analytic_kernel(
     const int idx = blockIdx.x * 256 + threadIdx.x;
     if (idx >= nrows * nsteps) return;
     const int r = idx / nsteps, k = idx % nsteps;
+    // the closed-form line's row coordinate uses the integer Ny/2 of the
+    // reference's analytic kernel (laue/kernels.py:172); the host takes
+    // this path only for even Ny, where both conventions agree
     const double dy = (double)(iy_lo + r - Ny / 2) * dx - lp.y0;
     const double dzv = (double)k * dz - lp.z0;
     const double d1 = dy * lp.e1y + dzv * lp.e1z;

@@ -105,7 +105,13 @@ def prepare_phase(seg_list, grid, xtal, *, H_gpu=None, precision=FP64,
     prec = _precision(precision)
     a = laue_args(grid, xtal)
     path, _, _, _ = plan(grid, xtal, prec)
-    ap = None if H_gpu is not None else analytic_line_params(seg_list, xtal)
+    # The reference's closed-form kernel places rows at (iy - Ny//2) dx
+    # (laue/kernels.py:172) and its H grid at (iy - Ny/2) dx
+    # (dislocations.py:146); they differ only for an odd Ny, which only the
+    # Ny= override can give, and then the Volterra table is used.
+    odd_rows = grid.geometry()[1] % 2 == 1
+    ap = (None if H_gpu is not None or odd_rows
+          else analytic_line_params(seg_list, xtal))
     if H_gpu is not None:
         H = N.asarray(H_gpu)
         if H.dtype not in (np.float64, np.float32):

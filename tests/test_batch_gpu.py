@@ -90,3 +90,70 @@ def test_groups_and_launch_shapes_change_nothing(dev, diamond400, x220):
     for a, b in zip(want, got):
         assert np.array_equal(a, b)
     small.close()
+
+
+def ensemble_subset(per_bucket=1, beam_width=1.0e-6, n_angles=2):
+    """Instances of the 256-instance ensemble draw (workloads.ensemble_batch)
+    that cover every compiled column count C from 8 to 18 (cfg4's
+    thicknesses, 50-300 um) for both reflections, each narrowed to a 1 um
+    beam (3 marched rows) and two of its own angles; the crystal,
+    thickness, x-grid and segments are the drawn ones."""
+    from dyfox_amd.laue.solver import plan
+    from dyfox_amd.workloads import ensemble_batch
+    picked, seen = [], {}
+    for i, (xtal, grid, segs, s) in enumerate(ensemble_batch(256)):
+        C = plan(grid, xtal)[1]
+        key = (C, i < 128)
+        if C < 8 or seen.get(key, 0) >= per_bucket:
+            continue
+        seen[key] = seen.get(key, 0) + 1
+        g = grid.replace(beam_width=beam_width)
+        pick = s[[0, len(s) // 2]][:n_angles]
+        picked.append((C, i, (xtal, g, segs, pick)))
+    return picked
+
+
+@pytest.mark.parametrize("prec", [FP64, MIXED], ids=["fp64", "mixed"])
+def test_ensemble_column_buckets_8_to_18_equal_single_solves(dev, prec):
+    """The persistent kernels of every column count the 256-instance
+    ensemble runs (C = 8, 10, 12, 13, 14, 16, 17, 18), through the
+    autotuned launch shape and graph replay: each instance equals its own
+    solve_dislocation (laue/solver.py:62-90) to 1e-13 relative."""
+    picked = ensemble_subset()
+    buckets = {C for C, _, _ in picked}
+    assert buckets == {8, 10, 12, 13, 14, 16, 17, 18}, buckets
+    assert len(picked) == 16
+    inst = [p[2] for p in picked]
+    solver = EnsembleSolver(inst, precision=prec)
+    solver.autotune(repeats=1)
+    outs = [to_numpy(o) for o in solver.run()]
+    for (C, i, (xtal, grid, segs, s)), got in zip(picked, outs):
+        print(f"ensemble instance {i} C={C} Nz={grid.Nz} "
+              f"Nx={grid.geometry()[0]} segs={len(segs)}", flush=True)
+        want = to_numpy(solve_dislocation(segs, s, grid, xtal,
+                                          precision=prec))
+        assert got.shape == want.shape and got.dtype == want.dtype
+        assert np.abs(got - want).max() <= 1e-13 * np.abs(want).max(), (C, i)
+    again = [to_numpy(o) for o in solver.run()]
+    for a, b in zip(outs, again):
+        assert np.array_equal(a, b)
+    solver.close()
+
+
+def test_ensemble_widest_buckets_match_oracle(dev):
+    """One C = 17 and one C = 18 instance of the ensemble draw against the
+    oracle (all rows of the narrowed beam, both angles): FP64 to 1e-10,
+    mixed to 1e-6."""
+    sub = ensemble_subset()
+    picked = [next(p for p in sub if p[0] == C) for C in (17, 18)]
+    inst = [p[2] for p in picked]
+    for prec, tol in ((FP64, 1e-10), (MIXED, 1e-6)):
+        solver = EnsembleSolver(inst, precision=prec)
+        outs = [to_numpy(o) for o in solver.run()]
+        for (C, i, (xtal, grid, segs, s)), got in zip(picked, outs):
+            want = oracle_exit(segs, s, grid, xtal)
+            for r in OL.error_report(np.asarray(got, np.complex128), want):
+                print(f"oracle C={C} instance {i} {prec.name} "
+                      f"rel_rms={r['rel_rms']:.2e}", flush=True)
+                assert r["rel_rms"] < tol and r["max_abs"] < tol, (C, i)
+        solver.close()

@@ -117,9 +117,10 @@ def test_surface_batch_matches_oracle(dev, diamond400, g):
     want, winfo = OB.solve_surface_batch(seg, sd, cfg, ys, diamond400, g)
     print(info, winfo)
     assert info["converged"] and info["residual"] < 1e-9
-    for key in ("residual", "iterations", "retries", "splits",
-                "unconverged_chunks", "converged"):
-        assert key in info
+    # the default is the reference's joint chunk: the same integer info
+    for key in ("iterations", "retries", "splits", "unconverged_chunks",
+                "converged"):
+        assert info[key] == winfo[key], key
     assert _rel(got, want) < 1e-8
     for j, y in enumerate(ys):
         one, oi = solve_bragg_stable(seg, sd, cfg, diamond400, g,
@@ -165,10 +166,22 @@ def test_joint_chunks_follow_scipy_and_agree_with_planes(dev, diamond400,
     want, winfo = OB.solve_surface_batch(seg, sd, cfg, ys, diamond400, g)
     assert ij["iterations"] == winfo["iterations"]
     assert _rel(joint, want) < 1e-8
-    ind, ii = solve_surface_batch(seg, sd, cfg, ys, diamond400, g)
+    ind, ii = solve_surface_batch(seg, sd, cfg, ys, diamond400, g,
+                                  planes="independent")
     assert ii["converged"] and ii["residual"] <= 1e-10
     assert ii["iterations"] <= ij["iterations"]
     assert _rel(ind, want) < 1e-8
+    # independent planes follow the reference's one-plane rule: at the
+    # default restart 40 the Krylov size is already the cap, so no retry
+    assert ii["retries"] == 0 and ii["splits"] == 0
+    # below the cap (restart 2, cap min(restart_cap, 8) = 8) a plane
+    # whose residual stays above `accept` is retried, never split
+    few, fi = solve_surface_batch(seg, sd, cfg, ys, diamond400, g,
+                                  planes="independent", restart=2,
+                                  per_plane=2, maxiter=1, grow=2, tries=3,
+                                  restart_cap=8)
+    assert fi["retries"] > 0 and fi["splits"] == 0
+    assert fi["converged"] == (fi["unconverged_chunks"] == 0)
 
 
 def test_plane_systems_equal_single_plane_solves(dev, diamond400, g):

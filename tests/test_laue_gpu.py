@@ -270,3 +270,29 @@ def test_wave_cache_round_trip(dev, diamond400, small_grid, tmp_path):
     second = cache.solve(seg, [0.0], small_grid, diamond400, tag="edge")
     assert first.dtype == np.complex128 and second.dtype == np.complex64
     assert np.allclose(to_numpy(second), to_numpy(first).astype(np.complex64))
+
+
+@pytest.mark.parametrize("line", ["inclined", "along_x"])
+def test_odd_row_count_agrees_across_paths(dev, diamond400, line):
+    """Ny = 33 (only the Ny= override gives an odd count): H is placed at
+    y = (iy - Ny/2) dx with a real Ny/2 as compute_H_grid places it
+    (dislocations.py:146), the field of view uses the march's integer
+    Ny/2 (laue/kernels.py:63).  The fused e^{iH} table, the H_gpu= path
+    and the oracle agree; a line along x (which would take the closed
+    form on an even grid) takes the Volterra table here."""
+    grid = LaueGrid(diamond400, 40e-6, 0.25e-6, 8e-6, Ny=33)
+    assert grid.shape[0] == 33
+    seg = (geometry_n(diamond400, grid, y0=grid.dx / 4) if line == "inclined"
+           else mixed_on_x(diamond400, grid))
+    s = _s(diamond400, [0.0, 117.5, -40.0])
+    fused = to_numpy(solve_dislocation(seg, s, grid, diamond400))
+    H = compute_H_grid(seg, grid, diamond400)
+    via_h = to_numpy(solve_dislocation(seg, s, grid, diamond400, H_gpu=H))
+    want = oracle_exit(seg, s, grid, diamond400)
+    _check(fused, want, 1e-10, "odd-Ny fused")
+    _check(via_h, want, 1e-10, "odd-Ny H_gpu")
+    # the same line on the even grid one row larger: rows move by half a
+    # pixel, so the answers differ (the convention is not a no-op)
+    even = LaueGrid(diamond400, 40e-6, 0.25e-6, 8e-6, Ny=34)
+    other = to_numpy(solve_dislocation(seg, s, even, diamond400))
+    assert np.abs(other[:, 1:33] - fused[:, 0:32]).max() > 1e-6
