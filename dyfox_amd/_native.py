@@ -133,6 +133,9 @@ _SIGNATURES = {
                                 C.POINTER(LineParams), vp, vp],
     "dyf_laue_march": [C.POINTER(LaueArgs), C.c_int, C.c_int, vp, C.c_int,
                        C.POINTER(LineParams), vp, C.c_int, vp, vp, vp],
+    "dyf_laue_march_noalloc": [C.POINTER(LaueArgs), C.c_int, C.c_int, vp,
+                               C.c_int, C.POINTER(LineParams), vp, C.c_int,
+                               vp, vp],
     "dyf_laue_batch_bytes": [C.POINTER(LaueInstance), C.c_int, C.c_int,
                              C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)],
     "dyf_laue_batch_prepare": [C.POINTER(LaueInstance), C.c_int, C.c_int, vp,
@@ -209,7 +212,8 @@ _initialised = False
 
 def declared_symbols():
     """Names this binding expects the library to export."""
-    return sorted(_SIGNATURES) + ["dyf_last_error"]
+    return sorted(_SIGNATURES) + ["dyf_last_error",
+                                  "dyf_laue_march_noalloc_max_bytes"]
 
 
 def load_library(path=LIB_PATH):
@@ -229,6 +233,8 @@ def load_library(path=LIB_PATH):
             fn.restype = C.c_int
         lib.dyf_last_error.argtypes = []
         lib.dyf_last_error.restype = C.c_char_p
+        lib.dyf_laue_march_noalloc_max_bytes.argtypes = []
+        lib.dyf_laue_march_noalloc_max_bytes.restype = C.c_size_t
         _lib = lib
         return lib
 

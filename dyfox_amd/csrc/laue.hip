@@ -25,6 +25,8 @@
 //
 // General path (any shift, any Nx): the reference's direct form, FP64, one
 // workgroup per (row, s_dev) with the rows double-buffered in LDS.
+#include <mutex>
+
 #include "internal.hpp"
 
 namespace dyf {
@@ -918,4 +920,79 @@ extern "C" int dyf_laue_march(const dyf_laue_args *a, int mode, int prec,
     }
     DYF_LAUNCHED("laue_general_kernel");
     return DYF_OK;
+}
+
+// ---- The march without a caller workspace ---------------------------------
+//
+// dyf_laue_march takes its per-(angle, step) coefficient table from a
+// workspace the caller allocates.  Callers that cannot allocate (code under
+// stream capture, tight serving loops) use dyf_laue_march_noalloc, which
+// keeps that table in `scratch`, a file-scope device array each GPU holds
+// for the life of the process.  Host code never passes `scratch` itself to a
+// kernel (on the host the name is only a stand-in for the device symbol);
+// the device address of each GPU's copy is looked up once per device with
+// hipGetSymbolAddress and kept in scratch_dev[].  The table is written in
+// full by laue_coef_kernel before the march reads it, so nothing needs
+// clearing between calls; calls that overlap on one GPU (different streams)
+// share the array and race.
+
+namespace dyf {
+
+constexpr size_t kNoAllocScratchBytes = size_t(4) << 20;   // 4 MiB
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+static std::mutex scratch_mu;
+static void *scratch_dev[kMaxDevices];
+
+// The device address of `scratch` on the current device (looked up on the
+// first call made with that device current).
+static int scratch_address(void **addr) {
+    int dev = -1;
+    DYF_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices)
+        return fail(DYF_EINVAL, "device %d outside the %d the library tracks",
+                    dev, kMaxDevices);
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return fail(DYF_EHIP, "hipGetSymbolAddress(scratch) on device "
+                        "%d: %s", dev, hipGetErrorString(e));
+        }
+        scratch_dev[dev] = p;
+    }
+    *addr = scratch_dev[dev];
+    return DYF_OK;
+}
+
+}  // namespace dyf
+
+extern "C" size_t dyf_laue_march_noalloc_max_bytes(void) {
+    return dyf::kNoAllocScratchBytes;
+}
+
+extern "C" int dyf_laue_march_noalloc(const dyf_laue_args *a, int mode,
+                                      int prec, const void *src, int H_prec,
+                                      const dyf_line_params *line,
+                                      const double *s_dev, int n_e,
+                                      void *out, void *stream) {
+    int rc = check_laue_args(a);
+    if (rc) return rc;
+    if (n_e < 0) return fail(DYF_EINVAL, "n_e < 0");
+    const size_t need = coef_bytes(n_e, a->Nz - 1);
+    if (need > kNoAllocScratchBytes)
+        return fail(DYF_EINVAL,
+                    "%d angles x %d steps need %zu bytes of coefficient "
+                    "scratch, above this entry point's %zu; march fewer "
+                    "angles per call or use dyf_laue_march with a workspace",
+                    n_e, a->Nz - 1, need, kNoAllocScratchBytes);
+    void *ws = nullptr;
+    rc = scratch_address(&ws);
+    if (rc) return rc;
+    return dyf_laue_march(a, mode, prec, src, H_prec, line, s_dev, n_e, ws,
+                          out, stream);
 }

@@ -151,6 +151,19 @@ int dyf_laue_march(const dyf_laue_args *a, int mode, int prec,
                    const void *src_dev, int H_prec,
                    const dyf_line_params *line, const double *s_dev_dev,
                    int n_e, void *coef_dev, void *out_dev, void *stream);
+/* dyf_laue_march without the coefficient workspace: the same kernels and
+ * results, with the per-(angle, step) table in a library-held device array
+ * of dyf_laue_march_noalloc_max_bytes() (4 MiB: n_e x (Nz-1) up to 87,381)
+ * per GPU, kept until the process ends.  Makes no allocation.  Larger
+ * calls fail with DYF_EINVAL; calls that overlap on one GPU race on the
+ * array.  No counterpart in the reference: it is for callers that cannot
+ * allocate (stream capture, serving loops, allocation-free harnesses).   */
+int dyf_laue_march_noalloc(const dyf_laue_args *a, int mode, int prec,
+                           const void *src_dev, int H_prec,
+                           const dyf_line_params *line,
+                           const double *s_dev_dev, int n_e, void *out_dev,
+                           void *stream);
+size_t dyf_laue_march_noalloc_max_bytes(void);
 
 /* ---- Batched Laue solves (persistent launch) -------------------------- */
 /* One instance: its grid and crystal, segments (host), angles and output.
