@@ -181,6 +181,8 @@ _SIGNATURES = {
                         vp],
     "dyf_bragg_precondition": [C.POINTER(BraggArgs), vp, C.c_int, vp, vp, vp,
                                vp],
+    "dyf_bragg_symbol_bytes": [C.POINTER(BraggArgs), C.c_int,
+                               C.POINTER(C.c_size_t)],
     "dyf_bragg_workspace_bytes": [C.POINTER(BraggArgs), C.c_int, C.c_int,
                                   C.POINTER(C.c_size_t)],
     "dyf_bragg_solve": [C.POINTER(BraggArgs), vp, C.c_int, vp, vp,

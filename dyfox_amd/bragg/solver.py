@@ -99,7 +99,9 @@ def precondition(cfg, xtal, s_dev, v):
     sd = N.DeviceArray.from_host(s)
     vd = N.DeviceArray.from_host(v)
     w = N.DeviceArray.empty(v.shape, np.complex128)
-    pinv = N.DeviceArray.empty((len(s), cfg.Nx), np.complex128)
+    nb = C.c_size_t()
+    N.call("dyf_bragg_symbol_bytes", C.byref(a), len(s), C.byref(nb))
+    pinv = N.DeviceArray.empty((nb.value,), np.uint8)
     N.call("dyf_bragg_precondition", C.byref(a), C.c_void_p(sd.ptr), len(s),
            C.c_void_p(vd.ptr), C.c_void_p(w.ptr), C.c_void_p(pinv.ptr), None)
     return w.get()

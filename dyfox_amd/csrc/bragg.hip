@@ -14,14 +14,12 @@
 // orthogonalisation is classical Gram-Schmidt applied twice (CGS2): two
 // batched reductions instead of SciPy's col+1 sequential MGS sweeps,
 // equally stable, and the only change is in rounding.
-#include <hipfft/hipfft.h>
-
 #include <algorithm>
 #include <map>
 #include <mutex>
 #include <vector>
 
-#include "common.hpp"
+#include "internal.hpp"
 
 namespace dyf {
 
@@ -591,59 +589,35 @@ __global__ __launch_bounds__(256) void bragg_symbol_kernel(
     }
 }
 
-__global__ __launch_bounds__(256) void spectrum_scale_kernel(
-    cd *__restrict__ w, const cd *__restrict__ pinv, int Nx, int nyc,
-    size_t n_total) {
-    for (size_t t = (size_t)blockIdx.x * 256 + threadIdx.x; t < n_total;
-         t += (size_t)gridDim.x * 256) {
-        const size_t s = t / ((size_t)nyc * Nx);
-        const int m = (int)(t % Nx);
-        w[t] = cmul(w[t], pinv[s * Nx + m]);
-    }
-}
-
-struct FftPlan {
-    hipfftHandle h;
-};
-static std::mutex g_fft_mu;
-static std::map<std::pair<int, int>, FftPlan> g_plans;
-
-static int fft_plan(int Nx, int batch, hipStream_t stream, hipfftHandle *out) {
-    std::lock_guard<std::mutex> lk(g_fft_mu);
-    auto key = std::make_pair(Nx, batch);
-    auto it = g_plans.find(key);
-    if (it == g_plans.end()) {
-        hipfftHandle h;
-        int n = Nx;
-        if (hipfftPlanMany(&h, 1, &n, nullptr, 1, Nx, nullptr, 1, Nx,
-                           HIPFFT_Z2Z, batch) != HIPFFT_SUCCESS)
-            return fail(DYF_EHIP, "hipfftPlanMany(n=%d, batch=%d) failed", Nx,
-                        batch);
-        it = g_plans.emplace(key, FftPlan{h}).first;
-    }
-    if (hipfftSetStream(it->second.h, stream) != HIPFFT_SUCCESS)
-        return fail(DYF_EHIP, "hipfftSetStream failed");
-    *out = it->second.h;
-    return DYF_OK;
-}
-
-// w = M v: forward FFT along x, divide by T22s (times 1/Nx), inverse FFT.
-static int apply_precon(const BraggK &k, const cd *pinv, cd *v, cd *w,
+// w = M v: the fused transform - scale - inverse transform of fft.hip on
+// every row (nsys systems of nyc planes), with the symbol written by
+// bragg_symbol_kernel (and, for convolution plans, its kernel spectrum).
+static int apply_precon(const BraggK &k, const cd *symbol, cd *v, cd *w,
                         hipStream_t stream) {
-    hipfftHandle h;
-    int rc = fft_plan(k.Nx, k.nsys * k.nyc, stream, &h);
+    FftPlan p;
+    int rc = fft_plan_for(k.Nx, &p);
     if (rc) return rc;
-    if (hipfftExecZ2Z(h, (hipfftDoubleComplex *)v, (hipfftDoubleComplex *)w,
-                      HIPFFT_FORWARD) != HIPFFT_SUCCESS)
-        return fail(DYF_EHIP, "hipfftExecZ2Z forward failed");
-    const size_t n = (size_t)k.nsys * k.nyc * k.Nx;
-    const int blocks = (int)std::min<size_t>((n + 255) / 256, 16384);
-    spectrum_scale_kernel<<<blocks, 256, 0, stream>>>(w, pinv, k.Nx, k.nyc, n);
-    DYF_LAUNCHED("spectrum_scale_kernel");
-    if (hipfftExecZ2Z(h, (hipfftDoubleComplex *)w, (hipfftDoubleComplex *)w,
-                      HIPFFT_BACKWARD) != HIPFFT_SUCCESS)
-        return fail(DYF_EHIP, "hipfftExecZ2Z backward failed");
-    return DYF_OK;
+    return fft_precondition(p, k.Nx, symbol, k.nsys, k.nyc, v, w, stream);
+}
+
+// The symbol of every system: 1 / (T22s Nx) per mode, then the
+// convolution plan's kernel spectrum where the plan has one.
+static int make_symbol(const BraggK &k, const double *s_dev, cd *symbol,
+                       hipStream_t stream) {
+    FftPlan p;
+    int rc = fft_plan_for(k.Nx, &p);
+    if (rc) return rc;
+    bragg_symbol_kernel<<<k.nsys, 256, 0, stream>>>(k, s_dev, symbol);
+    DYF_LAUNCHED("bragg_symbol_kernel");
+    return fft_prepare_spectrum(p, k.Nx, symbol, k.nsys, stream);
+}
+
+// Complex elements per system of the symbol (an upper bound when Nx has no
+// plan: the solve then fails before using the workspace).
+static size_t symbol_elems(int Nx) {
+    FftPlan p;
+    if (fft_plan_for(Nx, &p) != DYF_OK) return (size_t)Nx + 4096;
+    return fft_symbol_elems(p, Nx);
 }
 
 // ---- batched reductions ----------------------------------------------
@@ -1229,7 +1203,7 @@ static Work carve(void *base, int nsys, size_t N, int restart, int Nx) {
     w.h2 = (cd *)take(sizeof(cd) * nsys * (R + 1));
     w.nrm = (cd *)take(sizeof(cd) * nsys);
     w.nrm0 = (cd *)take(sizeof(cd) * nsys);
-    w.pinv = (cd *)take(sizeof(cd) * nsys * Nx);
+    w.pinv = (cd *)take(sizeof(cd) * nsys * symbol_elems(Nx));
     w.part = (cd *)take(sizeof(cd) * nsys * (R + 2) * P);
     w.inv = (double *)take(sizeof(double) * nsys);
     w.st = (SysState *)take(sizeof(SysState) * nsys);
@@ -1863,9 +1837,19 @@ extern "C" int dyf_bragg_precondition(const dyf_bragg_args *a,
     if (rc) return rc;
     hipStream_t stream = (hipStream_t)stream_;
     const BraggK k = make_k(a, nsys);
-    bragg_symbol_kernel<<<nsys, 256, 0, stream>>>(k, s_dev, (cd *)pinv_ws);
-    DYF_LAUNCHED("bragg_symbol_kernel");
+    if ((rc = make_symbol(k, s_dev, (cd *)pinv_ws, stream))) return rc;
     return apply_precon(k, (const cd *)pinv_ws, (cd *)v, (cd *)w, stream);
+}
+
+extern "C" int dyf_bragg_symbol_bytes(const dyf_bragg_args *a, int nsys,
+                                      size_t *bytes) {
+    int rc = check_bragg(a, nsys);
+    if (rc) return rc;
+    if (!bytes) return fail(DYF_EINVAL, "bytes is NULL");
+    FftPlan p;
+    if ((rc = fft_plan_for(a->Nx, &p))) return rc;
+    *bytes = sizeof(cd) * (size_t)nsys * fft_symbol_elems(p, a->Nx);
+    return DYF_OK;
 }
 
 extern "C" int dyf_bragg_workspace_bytes(const dyf_bragg_args *a, int nsys,
@@ -2028,8 +2012,7 @@ extern "C" int dyf_bragg_solve(const dyf_bragg_args *a, const double *s_dev,
 
     // preconditioner symbol for every system; b = -march(incident, 0);
     // ||b||, ||M b|| and ||r|| (r = b - A x0, or b) into the set-up
-    bragg_symbol_kernel<<<nsys, 256, 0, stream>>>(k, s_dev, W.pinv);
-    DYF_LAUNCHED("bragg_symbol_kernel");
+    if ((rc = make_symbol(k, s_dev, W.pinv, stream))) return rc;
     rc = launch_march(k, s_dev, E, (const cd *)incident, nullptr, 0, W.b, N,
                       -1.0, nullptr, stream);
     if (rc) return rc;

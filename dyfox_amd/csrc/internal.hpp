@@ -40,4 +40,32 @@ int launch_laue_persistent(const InstDesc *inst_dev, int n_inst,
                            unsigned *counters, int waves_per_block,
                            int blocks, hipStream_t stream);
 
+// ---- The Bragg preconditioner's transforms (fft.hip) -------------------
+constexpr int kFftMaxPasses = 16;
+
+// How rows of Nx points are transformed: at length N = Nx (conv = 0,
+// every prime factor of Nx <= 13) or as a linear convolution at a
+// 13-smooth N >= 2 Nx - 1 (conv = 1); radix[0..nst) are the passes.
+struct FftPlan {
+    int N, conv, nst;
+    int radix[kFftMaxPasses];
+};
+int fft_plan_for(int Nx, FftPlan *p);
+
+// Complex elements per system of the preconditioner symbol: the
+// 1 / (T22s Nx) row (Nx), then for convolution plans the kernel's
+// spectrum (N).
+size_t fft_symbol_elems(const FftPlan &p, int Nx);
+
+// For convolution plans: the kernel spectrum of every system from its
+// 1 / (T22s Nx) row (symbol layout above; nothing to do otherwise).
+int fft_prepare_spectrum(const FftPlan &p, int Nx, cx<double> *symbol,
+                         int nsys, hipStream_t stream);
+
+// w = ifft(fft(v) / T22s) on nsys * rows_per_sys rows of Nx points (row r
+// belongs to system r / rows_per_sys); v and w may be the same buffer.
+int fft_precondition(const FftPlan &p, int Nx, const cx<double> *symbol,
+                     int nsys, int rows_per_sys, const cx<double> *v,
+                     cx<double> *w, hipStream_t stream);
+
 }  // namespace dyf

@@ -253,10 +253,15 @@ int dyf_bragg_march(const dyf_bragg_args *a, const double *s_dev_dev,
                      const void *Dg0_dev, void *out_dev, void *stream);
 /* w = M v: ifft(fft(v)/T22s) per plane with the floored perfect-crystal
  * symbol (perfect_symbol + precon, bragg/solver.py:38-60, 163-168,
- * 272-275); pinv_dev receives 1/(T22s Nx), nsys x Nx complex.            */
+ * 272-275), one fused transform kernel; pinv_dev (dyf_bragg_symbol_bytes)
+ * receives 1/(T22s Nx) (nsys x Nx complex) and, when Nx has a prime factor
+ * above 13, the spectrum of the equivalent convolution after it.         */
 int dyf_bragg_precondition(const dyf_bragg_args *a, const double *s_dev_dev,
                            int nsys, const void *v_dev, void *w_dev,
                            void *pinv_dev, void *stream);
+/* Bytes of dyf_bragg_precondition's symbol workspace for nsys systems
+ * (the symbol the reference forms per call, bragg/solver.py:233-238).    */
+int dyf_bragg_symbol_bytes(const dyf_bragg_args *a, int nsys, size_t *bytes);
 int dyf_bragg_workspace_bytes(const dyf_bragg_args *a, int nsys, int restart,
                               size_t *bytes);
 /* Restarted, left-preconditioned GMRES on A v = -d for nsys systems at

@@ -68,6 +68,28 @@ def test_phase_march_and_preconditioner_match_oracle(dev, diamond400, g, rng,
         assert _rel(got_w, want_w) < 1e-12
 
 
+# Row lengths of every kind the preconditioner's transform plans take:
+# 13-smooth lengths transformed directly (324 = 2^2 3^4, 520 = 2^3 5 13,
+# 2048), and lengths with a larger prime factor taken as a linear
+# convolution at a 13-smooth length >= 2 Nx - 1 (124 = 4 x 31, 334 = 2 x
+# 167, 1031 prime, 1948 = 4 x 487: the thickest cfg5 grid, 37 prime).
+@pytest.mark.parametrize("Nx", [124, 324, 334, 520, 1031, 1948, 2048, 37])
+def test_fused_preconditioner_matches_numpy_fft(dev, diamond400, rng, Nx):
+    cfg = BraggConfig(diamond400, t_crystal=6e-6, dx=0.2e-6, half_x=2e-6)
+    cfg.Nx = Nx
+    s = deviation_from_urad(np.array([-6.5, 0.0, 12.0]), diamond400)
+    v = rng.normal(size=(3, 5, Nx)) + 1j * rng.normal(size=(3, 5, Nx))
+    got = BS.precondition(cfg, diamond400, s, v)
+    for j, sd in enumerate(s):
+        e0, eg = OB.step_constants(sd, complex(diamond400.sig0), cfg.ds)
+        _, T22 = OB.perfect_symbol(e0, eg, cfg.ds, cfg.Nz, Nx,
+                                   complex(diamond400.sig_h),
+                                   complex(diamond400.sig_hbar))
+        want = np.fft.ifft(np.fft.fft(v[j], axis=-1) / OB.floored_symbol(T22),
+                           axis=-1)
+        assert _rel(got[j], want) < 1e-12, (Nx, j, _rel(got[j], want))
+
+
 def test_one_and_two_wavefront_marches_agree(dev, diamond400, g, rng):
     # 41 angles x 100 planes = 4100 rows take the one-wavefront march (at
     # least 4096 rows); one angle (100 rows) takes the two-wavefront march,
