@@ -13,8 +13,14 @@ but the workspace buffers.
 `value` is FP64 pixel-steps/s; the same line carries the mixed
 (two-float FP32) rate on the same workload with its error against FP64,
 the worked-example drop-in rate, the roofline of the dominant kernel
-(the march, HIP events on the stream it runs on) and the CPU oracle's
-rate on a bounded sample of the same workload.
+(the march, HIP events on the stream it runs on), the Bragg legs (the
+reference's joint-chunk solve as the headline, independent planes
+beside it) with the roofline of the Bragg matvec, and the CPU oracle's
+rates on bounded samples of the same workloads (one process and all
+cores).
+
+    python bench.py --probe-laue   # the rocking-scan solves only (FP64
+                                   # then mixed), for a kernel trace
 """
 
 import argparse
@@ -31,8 +37,11 @@ sys.path.insert(0, ROOT)
 METRIC = ("Laue pixel-steps/s FP64 vs mixed @<=1e-6 rel err; "
           "Bragg solves/s cold vs warm")
 FLOPS_PER_PIXEL_STEP = 40        # 2x2 complex matvec 28 + two couplings 12
+ISSUED_FLOPS_PER_PIXEL_STEP = 32  # the kernel's 16 FP64 FMAs
+BRAGG_FLOPS_PER_PIXEL_STEP = 28   # the reference's _march_down step
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
+CPU_WORKERS_MAX = 16              # the GPU box's CPU share
 
 
 def parse():
@@ -48,6 +57,10 @@ def parse():
                     help="run only the 20 um Bragg surface batch with one edge line "
                          "one warm-up and five timed), for a kernel trace, "
                          "and exit")
+    ap.add_argument("--probe-laue", action="store_true",
+                    help="run only the rocking-scan solves (FP64 then "
+                         "mixed; one warm-up and five timed each), for a "
+                         "kernel trace, and exit")
     ap.add_argument("--probe", action="store_true",
                     help="for counter runs: one phase build and one march "
                          "per precision, then a 1 GiB copy kernel (the "
@@ -88,12 +101,12 @@ def time_laue(N, segs, s, grid, xtal, prec, steps, warmup):
     return wall, phase_ms, march_ms, out
 
 
-def time_bragg(xtal, crystal, repeats=3, planes="independent"):
+def time_bragg(xtal, crystal, repeats=3, planes="joint"):
     """20 um Bragg surface: 41 s_dev over one 32-plane chunk (Nz 90, Nx
-    520), solved as one batched
-    GMRES per repeat (cold start); returns solves/s and per-solve info.
-    planes="joint" is the reference's chunk (one Krylov space for the 32
-    planes of an angle), "independent" one Krylov space per plane."""
+    520), solved as one batched GMRES per repeat (cold start); returns
+    solves/s and per-solve info.  planes="joint" is the reference's chunk
+    (one Krylov space for the 32 planes of an angle, its iteration
+    counts), "independent" one Krylov space per plane."""
     from dyfox_amd.bragg import solve_surface_batch
     from dyfox_amd.workloads import bragg_surface
     cfg, segs, ys, s, g = bragg_surface(xtal, crystal)
@@ -238,27 +251,188 @@ def time_fixed_point(N, xtal):
             "cpu_oracle_solves_per_s": 1.0 / cpu}
 
 
-def cpu_baseline(segs, s, grid, xtal, rows=48, n_s=8):
-    """The NumPy FP64 oracle (Volterra H + direct-form march) on a bounded
-    sample: `rows` marched rows x `n_s` angles of the same workload."""
+def time_bragg_matvec(N, xtal, repeats=20):
+    """The GMRES matvec of the 20 um edge-line chunk on its own: one
+    down-march of D_g(0) (41 angles x 32 planes = 1312 rows, Nz 90, Nx
+    520; the kernel every iteration runs), HIP events around `repeats`
+    launches on the stream they run on."""
+    from dyfox_amd.bragg import solver as BS
+    from dyfox_amd.workloads import bragg_surface
+    cfg, segs, ys, s, g = bragg_surface(xtal, "edge_line")
+    E = BS.chunk_phase(segs, cfg, xtal, g, ys)
+    rng = np.random.default_rng(1)
+    v = rng.normal(size=(len(s), len(ys), cfg.Nx)) + 1j * rng.normal(
+        size=(len(s), len(ys), cfg.Nx))
+    import ctypes as C
+    a = BS.bragg_args(cfg, xtal, len(ys))
+    sd = N.DeviceArray.from_host(np.asarray(s, np.float64))
+    vd = N.DeviceArray.from_host(v)
+    out = N.DeviceArray.empty(v.shape, np.complex128)
+    args = (C.byref(a), C.c_void_p(sd.ptr), len(s), C.c_void_p(E.ptr), None,
+            C.c_void_p(vd.ptr), C.c_void_p(out.ptr), None)
+    N.call("dyf_bragg_march", *args)                         # warm-up
+    ev = [(N.Event(), N.Event()) for _ in range(repeats)]
+    N.synchronize()
+    for e0, e1 in ev:         # an event pair around each launch: the
+        e0.record()           # launch's own duration, host gaps excluded
+        N.call("dyf_bragg_march", *args)
+        e1.record()
+    ms = float(np.mean([e0.elapsed_ms(e1) for e0, e1 in ev]))
+    steps = len(s) * len(ys) * cfg.Nx * (cfg.Nz - 1)
+    # e^{iH} table read once (shared by the 41 angles), D_g(0) in, D_g(t) out
+    nbytes = cfg.Nz * len(ys) * cfg.Nx * 16 + 2 * len(s) * len(ys) * cfg.Nx * 16
+    tfl = BRAGG_FLOPS_PER_PIXEL_STEP * steps / (ms * 1e-3) / 1e12
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    return {"kernel": "bragg_march_halo_kernel<5, 3> (the GMRES matvec)",
+            "bound": "fp64_valu", "achieved": tfl, "peak": PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s", "frac": tfl / PEAK_FP64_TFLOPS,
+            "hbm_GB_per_s": gbs, "hbm_frac": gbs / PEAK_HBM_GBS,
+            "algorithmic_bytes": nbytes, "pixel_steps": steps,
+            "avg_launch_ms": ms,
+            "note": "28 algorithmic flops per pixel-step (the reference's "
+                    "_march_down step); the interaction-picture kernel "
+                    "issues 8 FP64 FMAs per pixel-step plus the halo "
+                    "lanes' duplicate columns"}
+
+
+def cpu_info():
+    """Host cores this process may use, the CPU model, library versions."""
+    import platform
+    import scipy
+    model = platform.processor() or "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:
+        nproc = os.cpu_count() or 1
+    return {"nproc": nproc, "cpu_model": model, "numpy": np.__version__,
+            "scipy": scipy.__version__,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
+def _laue_cpu_rows(args):
+    """Oracle Volterra H + direct-form march of the rocking scan on the
+    given marched rows and angle indices (one process); pixel-steps."""
+    pick, s_idx = args
+    from dyfox_amd.crystal import DIAMOND, CrystalParams
+    from dyfox_amd.workloads import laue_rocking_scan
     from oracle import laue as OL, volterra as OV
+    xtal = CrystalParams.from_reflection(DIAMOND, (4, 0, 0), 17.0,
+                                         f_prime=0.003, f_dprime=0.002)
+    grid, segs, s = laue_rocking_scan(xtal)
     Nx, Ny, dz, ds, spx = grid.geometry()
-    all_rows = grid.marched_rows()
-    pick = all_rows[np.linspace(0, len(all_rows) - 1, rows).astype(int)]
-    ss = np.asarray(s)[np.linspace(0, len(s) - 1, n_s).astype(int)]
-    t0 = time.perf_counter()
+    pick = np.asarray(pick)
     H = OV.h_grid(segs, xtal.g_vec, xtal.nu, xtal.a_core, Nx, Ny, grid.Nz,
                   grid.dx, dz, rows=pick)
-    OL.march(ss, complex(xtal.sig0), complex(xtal.sig_h),
-             complex(xtal.sig_hbar), grid.Nz, Nx, Ny, grid.dx, ds, spx,
-             grid.sheet_arg(), grid.beam_width / 2, H=H, rows=pick)
-    dt = time.perf_counter() - t0
-    work = rows * n_s * Nx * (grid.Nz - 1)
-    return {"value": work / dt, "unit": "pixel-steps/s", "cores": 1,
+    OL.march(np.asarray(s)[list(s_idx)], complex(xtal.sig0),
+             complex(xtal.sig_h), complex(xtal.sig_hbar), grid.Nz, Nx, Ny,
+             grid.dx, ds, spx, grid.sheet_arg(), grid.beam_width / 2, H=H,
+             rows=pick)
+    return len(pick) * len(s_idx) * Nx * (grid.Nz - 1)
+
+
+_THREAD_VARS = ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS")
+
+
+def _warm_worker(_):
+    import oracle.bragg  # noqa: F401
+    import oracle.laue  # noqa: F401
+    import scipy.sparse.linalg  # noqa: F401
+    return 0
+
+
+def _bragg_cpu_angle(j):
+    """The reference's CPU path (NumPy march + SciPy GMRES, joint chunk)
+    for angle j of the 20 um edge-line surface; GMRES iterations."""
+    from dyfox_amd.crystal import DIAMOND, CrystalParams
+    from dyfox_amd.workloads import bragg_surface
+    from oracle import bragg as OB
+    xtal = CrystalParams.from_reflection(DIAMOND, (4, 0, 0), 17.0,
+                                         f_prime=0.003, f_dprime=0.002)
+    cfg, segs, ys, s, g = bragg_surface(xtal, "edge_line")
+    _, info = OB.solve_surface_batch(segs, float(s[j]), cfg, ys, xtal, g)
+    return info["iterations"]
+
+
+def cpu_baseline(rows=48, n_s=8):
+    """The NumPy FP64 oracle on bounded samples of the benchmark workloads,
+    on the host cores, one process and all cores (a process pool, one
+    row block x angle set or one Bragg angle per task).  Run before the
+    GPU is initialised (the pool's workers are fresh interpreters).
+
+    Laue: the rocking scan's Volterra H and direct-form march on `rows`
+    marched rows x `n_s` angles (one process), and on W times that many
+    rows over W workers.  Bragg: the 20 um edge-line surface chunk
+    (reference CPU path, joint chunk) for 4 angles in one process, and
+    all 41 angles over the workers."""
+    import multiprocessing as mp
+    from dyfox_amd.crystal import DIAMOND, CrystalParams
+    from dyfox_amd.workloads import laue_rocking_scan
+    info = cpu_info()
+    workers = max(1, min(CPU_WORKERS_MAX, info["nproc"]))
+    xtal = CrystalParams.from_reflection(DIAMOND, (4, 0, 0), 17.0,
+                                         f_prime=0.003, f_dprime=0.002)
+    grid, _, s = laue_rocking_scan(xtal)
+    marched = grid.marched_rows()
+    s_idx = tuple(np.linspace(0, len(s) - 1, n_s).astype(int).tolist())
+    one_rows = marched[np.linspace(0, len(marched) - 1, rows).astype(int)]
+    t0 = time.perf_counter()
+    work1 = _laue_cpu_rows((one_rows.tolist(), s_idx))
+    dt1 = time.perf_counter() - t0
+    blocks = np.array_split(marched[:min(len(marched), rows * workers)],
+                            workers)
+    ctx = mp.get_context("spawn")
+    # one thread per worker process (the workers inherit this environment)
+    saved = {k: os.environ.get(k) for k in _THREAD_VARS}
+    os.environ.update({k: "1" for k in _THREAD_VARS})
+    try:
+        pool = ctx.Pool(workers)
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    with pool:
+        pool.map(_warm_worker, range(workers))          # start-up, imports
+        t0 = time.perf_counter()
+        workn = sum(pool.map(_laue_cpu_rows,
+                             [(b.tolist(), s_idx) for b in blocks]))
+        dtn = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        its_all = pool.map(_bragg_cpu_angle, list(range(41)))
+        dtb_n = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for j in (0, 13, 27, 40):
+        _bragg_cpu_angle(j)
+    dtb_1 = time.perf_counter() - t0
+    return {"value": workn / dtn, "unit": "pixel-steps/s", "cores": workers,
             "kind": "port",
-            "sample": f"{rows} marched rows x {n_s} s_dev of the same "
-                      f"workload ({work:.3e} pixel-steps, Volterra H of "
-                      f"those rows included), {dt:.1f} s, 1 process"}
+            "sample": f"rocking scan: {rows * workers} marched rows x {n_s} "
+                      f"s_dev ({workn:.3e} pixel-steps, Volterra H of those "
+                      f"rows included) over {workers} processes in "
+                      f"{dtn:.1f} s",
+            "single_process": {"value": work1 / dt1, "cores": 1,
+                               "sample": f"{rows} rows x {n_s} s_dev "
+                                         f"({work1:.3e} pixel-steps), "
+                                         f"{dt1:.1f} s"},
+            "bragg_surface_20um_edge_line": {
+                "unit": "solves/s",
+                "all_cores": 41 / dtb_n, "cores": workers,
+                "single_process": 4 / dtb_1,
+                "gmres_iterations": [min(its_all), max(its_all)],
+                "sample": "reference CPU path (NumPy _march_down, NumPy FFT "
+                          "preconditioner, SciPy GMRES, joint 32-plane "
+                          "chunk): 41 angles over the workers, 4 in one "
+                          "process"},
+            "host": info}
 
 
 def warm_start_report():
@@ -322,6 +496,24 @@ def probe():
     print("probe done", flush=True)
 
 
+def probe_laue():
+    """The rocking-scan solves alone (FP64, then mixed; one warm-up and
+    five timed each), so a kernel trace's averages are this workload's."""
+    from dyfox_amd import _native as N
+    from dyfox_amd.backend import FP64, MIXED, device
+    from dyfox_amd.crystal import DIAMOND, CrystalParams
+    from dyfox_amd.workloads import laue_rocking_scan
+    device()
+    xtal = CrystalParams.from_reflection(DIAMOND, (4, 0, 0), 17.0,
+                                         f_prime=0.003, f_dprime=0.002)
+    grid, segs, s = laue_rocking_scan(xtal)
+    for prec in (FP64, MIXED):
+        w, p, m, out = time_laue(N, segs, s, grid, xtal, prec, 5, 1)
+        del out
+        print(f"probe-laue {prec.name}: step {w * 1e3:.2f} ms, phase "
+              f"{p:.2f} ms, march {m:.2f} ms", flush=True)
+
+
 def probe_bragg():
     """The 20 um Bragg surface batch with one edge line, independent
     planes: one warm-up batch, five timed."""
@@ -341,6 +533,8 @@ def main():
         return probe()
     if args.probe_bragg:
         return probe_bragg()
+    if args.probe_laue:
+        return probe_laue()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: there is no multi-GPU path here "
               "(the work would shard over rows x s_dev with no "
@@ -352,6 +546,13 @@ def main():
     from dyfox_amd.workloads import laue_rocking_scan, laue_worked_example
     from oracle.laue import error_report
 
+    # the CPU baseline first: its worker processes start before this
+    # process initialises the GPU
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline()
+        print(f"[bench] cpu baseline {cpu['value']:.3e} pixel-steps/s on "
+              f"{cpu['cores']} processes", file=sys.stderr, flush=True)
     device()
     xtal = CrystalParams.from_reflection(DIAMOND, (4, 0, 0), 17.0,
                                          f_prime=0.003, f_dprime=0.002)
@@ -387,22 +588,35 @@ def main():
           flush=True)
 
     bragg = {c: time_bragg(xtal, c) for c in ("perfect", "edge_line")}
-    bragg["edge_line_joint_chunk"] = time_bragg(xtal, "edge_line",
-                                                planes="joint")
+    bragg["edge_line_independent_planes"] = time_bragg(
+        xtal, "edge_line", planes="independent")
+    bragg["semantics"] = ("perfect and edge_line: the reference's joint "
+                          "32-plane chunk (its iterations, retries, splits); "
+                          "edge_line_independent_planes: opt-in, one Krylov "
+                          "space per plane")
     print(f"[bench] bragg surface, edge line: "
-          f"{bragg['edge_line']['solves_per_s']:.1f} solves/s (joint chunk "
-          f"{bragg['edge_line_joint_chunk']['solves_per_s']:.1f})",
+          f"{bragg['edge_line']['solves_per_s']:.1f} solves/s (independent "
+          f"planes {bragg['edge_line_independent_planes']['solves_per_s']:.1f})",
           file=sys.stderr, flush=True)
+    bragg_roof = time_bragg_matvec(N, xtal)
 
     achieved = FLOPS_PER_PIXEL_STEP * work / (m64 * 1e-3) / 1e12
-    roof = {"kernel": "laue_fast_kernel<C=17, GRID, FP64>",
-            "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-            "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
+    issued = ISSUED_FLOPS_PER_PIXEL_STEP * work / (m64 * 1e-3) / 1e12
+    roof = {"kernel": "laue_fast_kernel<17, GRID, FP64>",
+            "bound": "fp64_valu", "achieved": achieved,
+            "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP64_TFLOPS,
             "traffic": traffic_from_profile(),
+            "traffic_source": "profiles/laue_march_rocking_scan_273um_"
+                              "traffic.md (rocprofv3 --pmc FETCH_SIZE, "
+                              "WRITE_SIZE; scaled by the copy-kernel "
+                              "calibration)",
+            "issued_tflops": issued,
+            "issued_frac": issued / PEAK_FP64_TFLOPS,
             "note": "FP64 vector-ALU bound (no MFMA: the march is not a "
-                    "contraction); MI355X's FP64 vector and FP64 matrix "
-                    "peaks are both 78.6 TFLOP/s. 40 algorithmic flops per "
-                    "pixel-step; the kernel itself issues 16 FP64 FMAs",
+                    "contraction). achieved counts the 40 algorithmic "
+                    "flops per pixel-step; the interaction-picture kernel "
+                    "issues 16 FP64 FMAs (32 flops), issued_* counts those",
             "avg_launch_ms": m64}
     line = {
         "metric": METRIC, "value": work / w64, "unit": "pixel-steps/s",
@@ -434,11 +648,14 @@ def main():
         "laue_ensemble_256": None if args.no_ensemble else time_ensembles(),
         "bragg_warm_start_20k": warm_start_report(),
         "roofline": roof,
+        "bragg_roofline": bragg_roof,
     }
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(segs, s, grid, xtal)
-        line["cpu_baseline"]["gpu_over_cpu"] = line["value"] / \
-            line["cpu_baseline"]["value"]
+    if cpu is not None:
+        cpu["gpu_over_cpu"] = line["value"] / cpu["value"]
+        cb = cpu["bragg_surface_20um_edge_line"]
+        cb["gpu_over_cpu"] = bragg["edge_line"]["solves_per_s"] / \
+            cb["all_cores"]
+        line["cpu_baseline"] = cpu
     print(json.dumps(line))
 
 
