@@ -61,6 +61,11 @@ def parse():
                     help="run only the rocking-scan solves (FP64 then "
                          "mixed; one warm-up and five timed each), for a "
                          "kernel trace, and exit")
+    ap.add_argument("--probe-gemm", action="store_true",
+                    help="time the warm-start network's GEMMs (forward, "
+                         "data and weight gradients of every 3x3 layer at "
+                         "batch 32) with HIP events, print one line each, "
+                         "and exit")
     ap.add_argument("--probe", action="store_true",
                     help="for counter runs: one phase build and one march "
                          "per precision, then a 1 GiB copy kernel (the "
@@ -514,6 +519,59 @@ def probe_laue():
               f"{p:.2f} ms, march {m:.2f} ms", flush=True)
 
 
+def probe_gemm(batch=32, repeats=10):
+    """Every 3x3 layer's three products as the network issues them (the
+    operand layouts and the weight gradient's pixel split of
+    warmstart/net.py), on uninitialised operands, timed with an event
+    pair around each launch."""
+    import ctypes as C
+    from dyfox_amd import _native as N
+    from dyfox_amd.backend import device
+    from dyfox_amd.warmstart.net import BUFFERS, H0, SPEC, W0
+    device()
+    peak_bf16 = 2500.0                       # TFLOP/s, dense
+    for op in SPEC:
+        if op[0] != "conv3":
+            continue
+        lvl = BUFFERS[op[1]][0]
+        P = batch * (H0 >> lvl) * (W0 >> lvl)
+        cin, cout = op[3], op[6]
+        K9 = 9 * cin
+        col = N.DeviceArray.empty((P * K9,), np.uint16)
+        W = N.DeviceArray.empty((cout * K9,), np.uint16)
+        dz = N.DeviceArray.empty((P * cout,), np.uint16)
+        out = N.DeviceArray.empty((P * K9 * 2,), np.uint16)
+        splits = max(1, min(256, P // 1024))
+        v = C.c_void_p
+        products = [
+            ("fwd", P, cout, K9, (v(col.ptr), K9, 1, v(W.ptr), K9, 1, P, cout,
+                                  K9, 0, 1, None, v(out.ptr), cout, 1, None)),
+            ("dgrad", P, K9, cout, (v(dz.ptr), cout, 1, v(W.ptr), 1, K9, P, K9,
+                                    cout, 1, 0, None, v(out.ptr), K9, 1,
+                                    None)),
+            ("wgrad", cout, K9, P, (v(dz.ptr), 1, cout, v(col.ptr), 1, K9,
+                                    cout, K9, P, 2, 0, None, v(out.ptr), K9,
+                                    splits, None))]
+        for name, M, Nn, K, args in products:
+            N.call("dyf_nn_gemm", *args)
+            ev = [(N.Event(), N.Event()) for _ in range(repeats)]
+            N.synchronize()
+            for e0, e1 in ev:
+                e0.record()
+                N.call("dyf_nn_gemm", *args)
+                e1.record()
+            ms = float(np.mean([e0.elapsed_ms(e1) for e0, e1 in ev]))
+            fl = 2.0 * M * Nn * K
+            nb = 2.0 * (M * K + Nn * K) + (2 if name == "fwd" else 4) * M * Nn
+            print(f"{name:5s} conv3 {cin:3d}->{cout:2d} level {lvl} M={M:7d} "
+                  f"N={Nn:4d} K={K:7d}: {ms * 1e3:7.1f} us "
+                  f"{fl / ms / 1e9:6.1f} TFLOP/s "
+                  f"({fl / ms / 1e9 / peak_bf16 * 100:5.2f}% of BF16) "
+                  f"{nb / ms / 1e6:6.0f} GB/s "
+                  f"({nb / ms / 1e6 / PEAK_HBM_GBS * 100:5.1f}% of HBM)",
+                  flush=True)
+
+
 def probe_bragg():
     """The 20 um Bragg surface batch with one edge line, independent
     planes: one warm-up batch, five timed."""
@@ -535,6 +593,8 @@ def main():
         return probe_bragg()
     if args.probe_laue:
         return probe_laue()
+    if args.probe_gemm:
+        return probe_gemm()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: there is no multi-GPU path here "
               "(the work would shard over rows x s_dev with no "

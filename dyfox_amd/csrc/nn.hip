@@ -32,11 +32,21 @@ __device__ __forceinline__ bf16 f2bf(float f) {   // round to nearest even
 }
 
 // ---- GEMM: C(i,j) = sum_k A(i,k) B(j,k), BF16 in, FP32 accumulate ------
-// A(i,k) = A[i*a_i + k*a_k], B(j,k) = B[j*b_j + k*b_k].  One 256-thread
-// workgroup computes a 64x64 tile; wave w owns the 32x32 quarter
-// (w/2, w%2) as 2x2 MFMA 16x16x16 tiles.  grid.z splits K.
-// Epilogue: 0 = BF16 store of relu?(acc + bias) into C(i,j) = C[i*c_i + j],
-//           1 = FP32 store (+bias), 2 = FP32 atomic add.
+// A(i,k) = A[i*a_i + k*a_k], B(j,k) = B[j*b_j + k*b_k].  A 256-thread
+// workgroup computes a BM x BN tile; its four waves are laid out WR x
+// (4/WR), each owning (BM/WR) x (BN*WR/4) as 16x16 tiles of gfx950's
+// v_mfma_f32_16x16x32_bf16 (K = 32 per instruction; lane l supplies
+// row l%16, k = 8(l/16) .. 8(l/16)+7 of A and likewise of B, and holds
+// column l%16, rows 4(l/16) .. 4(l/16)+3 of the result).  BN follows N
+// (16, 32 or 64 columns), so the narrow layers of the network do not
+// pay for a 64-wide tile.  K steps of 32 are staged through LDS; each
+// operand is loaded from global memory according to its layout:
+//   k-contiguous (stride 1 along k): 16-byte loads of 8 k values,
+//   row-contiguous (stride 1 along i or j): 16-byte loads of 8 rows at
+//     one k, written to LDS transposed,
+//   anything else (or misaligned): one element per load.
+// grid.z splits K.  Epilogue: 0 = BF16 store of relu?(acc + bias) into
+// C(i,j) = C[i*c_i + j], 1 = FP32 store (+bias), 2 = FP32 atomic add.
 struct GemmArgs {
     const bf16 *A;
     long long a_i, a_k;
@@ -49,71 +59,109 @@ struct GemmArgs {
     long long c_i;
 };
 
-constexpr int TM = 64, TN = 64, TK = 32, LDK = TK + 8;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
+constexpr int TK = 32, LDK = TK + 8;
+enum { LOAD_ANY = 0, LOAD_K = 1, LOAD_ROW = 2 };
+
+// Stage rows [r0, r0 + R) x k [k0, k0 + TK) of X(r, k) = X[r*s_r + k*s_k]
+// into S[r - r0][k - k0] (zero outside r < rows, k < ke).
+template <int R, int MODE>
+__device__ __forceinline__ void stage(bf16 *__restrict__ S,
+                                      const bf16 *__restrict__ X,
+                                      long long s_r, long long s_k, int r0,
+                                      int rows, int k0, int ke, int tid) {
+    if constexpr (MODE == LOAD_K) {
+        // R x TK / 8 vectors, 4 per row
+        for (int v = tid; v < R * (TK / 8); v += 256) {
+            const int r = v >> 2, kq = (v & 3) * 8;
+            const int gr = r0 + r, gk = k0 + kq;
+            uint4 q = {0u, 0u, 0u, 0u};
+            if (gr < rows && gk + 7 < ke) {
+                q = *(const uint4 *)(X + (long long)gr * s_r + gk);
+            } else if (gr < rows) {
+                unsigned short e[8];
+#pragma unroll
+                for (int t = 0; t < 8; ++t)
+                    e[t] = gk + t < ke ? X[(long long)gr * s_r + gk + t] : 0;
+                q = {e[0] | (unsigned)e[1] << 16, e[2] | (unsigned)e[3] << 16,
+                     e[4] | (unsigned)e[5] << 16, e[6] | (unsigned)e[7] << 16};
+            }
+            *(uint4 *)(S + r * LDK + kq) = q;
+        }
+    } else if constexpr (MODE == LOAD_ROW) {
+        // (R / 8) row groups x TK k values
+        for (int v = tid; v < (R / 8) * TK; v += 256) {
+            const int k = v % TK, rq = (v / TK) * 8;
+            const int gr = r0 + rq, gk = k0 + k;
+            unsigned short e[8];
+            if (gk < ke && gr + 7 < rows) {
+                const uint4 q = *(const uint4 *)(X + gr + (long long)gk * s_k);
+                e[0] = q.x & 0xffff, e[1] = q.x >> 16, e[2] = q.y & 0xffff,
+                e[3] = q.y >> 16, e[4] = q.z & 0xffff, e[5] = q.z >> 16,
+                e[6] = q.w & 0xffff, e[7] = q.w >> 16;
+            } else {
+#pragma unroll
+                for (int t = 0; t < 8; ++t)
+                    e[t] = (gk < ke && gr + t < rows)
+                               ? X[gr + t + (long long)gk * s_k] : 0;
+            }
+#pragma unroll
+            for (int t = 0; t < 8; ++t) S[(rq + t) * LDK + k] = e[t];
+        }
+    } else {
+        for (int v = tid; v < R * TK; v += 256) {
+            const int r = v / TK, k = v % TK;
+            const int gr = r0 + r, gk = k0 + k;
+            S[r * LDK + k] = (gr < rows && gk < ke)
+                                 ? X[(long long)gr * s_r + (long long)gk * s_k]
+                                 : (bf16)0;
+        }
+    }
+}
+
+template <int BM, int BN, int WR, int LA, int LB>
 __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
-    __shared__ bf16 As[TM * LDK];
-    __shared__ bf16 Bs[TN * LDK];
+    constexpr int WC = 4 / WR;
+    constexpr int RT = BM / WR / 16, CT = BN / WC / 16;
+    static_assert(RT >= 1 && CT >= 1, "tile too small for the wave grid");
+    __shared__ __align__(16) bf16 As[BM * LDK];
+    __shared__ __align__(16) bf16 Bs[BN * LDK];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int i0 = blockIdx.x * TM, j0 = blockIdx.y * TN;
+    const int i0 = blockIdx.x * BM, j0 = blockIdx.y * BN;
     const int kb = blockIdx.z * g.k_chunk;
     const int ke = min(g.K, kb + g.k_chunk);
-    float4_t acc[2][2];
+    const int wr = (wave / WC) * (BM / WR), wc = (wave % WC) * (BN / WC);
+    float4_t acc[RT][CT];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < RT; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = float4_t{0.f, 0.f, 0.f, 0.f};
-    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-    // each thread stages 8 consecutive k of one row of A and of B
-    const int lr = tid >> 2, lk = (tid & 3) * 8;
+        for (int b = 0; b < CT; ++b) acc[a][b] = float4_t{0.f, 0.f, 0.f, 0.f};
+    const int ko = 8 * (lane >> 4), lr = lane & 15;
     for (int k0 = kb; k0 < ke; k0 += TK) {
-        {
-            const int i = i0 + lr;
-#pragma unroll
-            for (int t = 0; t < 8; ++t) {
-                const int k = k0 + lk + t;
-                As[lr * LDK + lk + t] =
-                    (i < g.M && k < ke) ? g.A[(long long)i * g.a_i + (long long)k * g.a_k]
-                                        : (bf16)0;
-            }
-            const int j = j0 + lr;
-#pragma unroll
-            for (int t = 0; t < 8; ++t) {
-                const int k = k0 + lk + t;
-                Bs[lr * LDK + lk + t] =
-                    (j < g.N && k < ke) ? g.B[(long long)j * g.b_j + (long long)k * g.b_k]
-                                        : (bf16)0;
-            }
-        }
+        stage<BM, LA>(As, g.A, g.a_i, g.a_k, i0, g.M, k0, ke, tid);
+        stage<BN, LB>(Bs, g.B, g.b_j, g.b_k, j0, g.N, k0, ke, tid);
         __syncthreads();
+        bf16x8 af[RT], bfr[CT];
 #pragma unroll
-        for (int kk = 0; kk < TK; kk += 16) {
-            const int ko = kk + 4 * (lane >> 4);
-            short4_t af[2], bfr[2];
+        for (int a = 0; a < RT; ++a)
+            af[a] = *(const bf16x8 *)&As[(wr + 16 * a + lr) * LDK + ko];
 #pragma unroll
-            for (int a = 0; a < 2; ++a) {
-                const bf16 *p = &As[(wr + 16 * a + (lane & 15)) * LDK + ko];
-                af[a] = short4_t{(short)p[0], (short)p[1], (short)p[2], (short)p[3]};
-            }
+        for (int b = 0; b < CT; ++b)
+            bfr[b] = *(const bf16x8 *)&Bs[(wc + 16 * b + lr) * LDK + ko];
 #pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                const bf16 *p = &Bs[(wc + 16 * b + (lane & 15)) * LDK + ko];
-                bfr[b] = short4_t{(short)p[0], (short)p[1], (short)p[2], (short)p[3]};
-            }
+        for (int a = 0; a < RT; ++a)
 #pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-                for (int b = 0; b < 2; ++b)
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
-                        af[a], bfr[b], acc[a][b], 0, 0, 0);
-        }
+            for (int b = 0; b < CT; ++b)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    af[a], bfr[b], acc[a][b], 0, 0, 0);
         __syncthreads();
     }
     // lane holds column (lane & 15), rows 4*(lane >> 4) + t of each tile
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < RT; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < CT; ++b)
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
                 const int i = i0 + wr + 16 * a + 4 * (lane >> 4) + t;
@@ -377,6 +425,44 @@ static int grid_for(long long n) {
 
 using namespace dyf;
 
+// How an operand X(r, k) = X[r*s_r + k*s_k] is staged: 16-byte loads
+// along k when k is contiguous and rows start 16-byte aligned, along r
+// when r is contiguous and k columns start aligned, else element loads.
+static int load_mode(const void *X, long long s_r, long long s_k) {
+    const bool base = ((uintptr_t)X & 15) == 0;
+    if (base && s_k == 1 && s_r % 8 == 0) return LOAD_K;
+    if (base && s_r == 1 && s_k % 8 == 0) return LOAD_ROW;
+    return LOAD_ANY;
+}
+
+using GemmFn = void (*)(GemmArgs);
+
+template <int BM, int BN, int WR>
+static GemmFn gemm_for_loads(int la, int lb) {
+#define DYF_GEMM_LB(LA)                                               \
+    lb == LOAD_K ? gemm_bf16_kernel<BM, BN, WR, LA, LOAD_K>           \
+    : lb == LOAD_ROW ? gemm_bf16_kernel<BM, BN, WR, LA, LOAD_ROW>     \
+                     : gemm_bf16_kernel<BM, BN, WR, LA, LOAD_ANY>
+    if (la == LOAD_K) return DYF_GEMM_LB(LOAD_K);
+    if (la == LOAD_ROW) return DYF_GEMM_LB(LOAD_ROW);
+    return DYF_GEMM_LB(LOAD_ANY);
+#undef DYF_GEMM_LB
+}
+
+// tiles: 128 x {16, 32, 64} (waves stacked along i) for the tall products
+// of the forward and data-gradient passes, 64 x {16, 32, 64} (2 x 2
+// waves, or 4 along i for narrow tiles) for the weight gradients
+static GemmFn pick_gemm(int bm, int bn, int la, int lb) {
+    if (bm == 128) {
+        if (bn == 16) return gemm_for_loads<128, 16, 4>(la, lb);
+        if (bn == 32) return gemm_for_loads<128, 32, 4>(la, lb);
+        return gemm_for_loads<128, 64, 4>(la, lb);
+    }
+    if (bn == 16) return gemm_for_loads<64, 16, 4>(la, lb);
+    if (bn == 32) return gemm_for_loads<64, 32, 2>(la, lb);
+    return gemm_for_loads<64, 64, 2>(la, lb);
+}
+
 extern "C" {
 
 int dyf_nn_gemm(const void *A, long long a_i, long long a_k, const void *B,
@@ -395,8 +481,12 @@ int dyf_nn_gemm(const void *A, long long a_i, long long a_k, const void *B,
     g.k_chunk = ((K + k_splits - 1) / k_splits + TK - 1) / TK * TK;
     g.mode = mode, g.relu = relu, g.bias = bias, g.C = C, g.c_i = c_i;
     const int splits = (K + g.k_chunk - 1) / g.k_chunk;
-    dim3 grid((M + TM - 1) / TM, (N + TN - 1) / TN, splits);
-    gemm_bf16_kernel<<<grid, 256, 0, (hipStream_t)stream>>>(g);
+    const int la = load_mode(A, a_i, a_k), lb = load_mode(B, b_j, b_k);
+    const int bm = M >= 4096 ? 128 : 64;
+    const int bn = N <= 16 ? 16 : N <= 32 ? 32 : 64;
+    GemmFn fn = pick_gemm(bm, bn, la, lb);
+    dim3 grid((M + bm - 1) / bm, (N + bn - 1) / bn, splits);
+    hipLaunchKernelGGL(fn, grid, dim3(256), 0, (hipStream_t)stream, g);
     DYF_LAUNCHED("gemm_bf16_kernel");
     return DYF_OK;
 }

@@ -209,7 +209,10 @@ class WarmStartNet:
                 N.call("dyf_nn_relu_grad", _vp(self.grad[dst]),
                        _vp(self.act[dst]) if dst != "out" else _vp(self.grad[dst]),
                        P, dct, doff, cout, relu, _vp(self.dz), None)
-                splits = max(1, min(64, P // 4096))
+                # split the pixel sum: 1024 pixels per workgroup, at most
+                # 256 splits (measured: 1.86 -> 0.65 ms per backward pass
+                # for the 14 3x3 layers at batch 32, against 4096 / 64)
+                splits = max(1, min(256, P // 1024))
                 if kind == "conv3":
                     col = self.cols[i]
                     N.call("dyf_nn_gemm", _vp(self.dz), 1, cout, _vp(col), 1,

@@ -27,9 +27,16 @@ def bf(a):
     return from_bf16_bits(to_bf16_bits(a)).astype(np.float64)
 
 
+# element loads (300, 517, M = 33), 16-byte loads along k with a k tail
+# (264) and along rows (trans, M and N multiples of 8), the 128-row tiles
+# (M >= 4096) and each column-tile width (16, 32, 64)
 @pytest.mark.parametrize("M,N_,K,trans", [(70, 45, 300, False),
                                           (128, 64, 32, False),
-                                          (33, 200, 517, True)])
+                                          (33, 200, 517, True),
+                                          (200, 48, 264, False),
+                                          (96, 24, 1000, True),
+                                          (4100, 16, 72, False),
+                                          (4104, 40, 96, True)])
 def test_gemm_matches_a_float64_product(dev, rng, M, N_, K, trans):
     A = rng.normal(size=(M, K)).astype(np.float32)
     B = rng.normal(size=(N_, K)).astype(np.float32)
