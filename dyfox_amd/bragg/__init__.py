@@ -18,9 +18,10 @@ from .geometry import (BraggConfig, bragg_g_vector, edge_segment,
 from .reference import (darwin_width_urad, reflectivity_riccati,
                         reflectivity_semi_infinite)
 from .solver import (perfect_symbol, solve_bragg, solve_bragg_stable,
-                     solve_surface_batch)
+                     solve_surface_batch, solve_surface_fields)
 
 __all__ = ["BraggConfig", "bragg_g_vector", "edge_segment",
            "threading_segment", "darwin_width_urad", "reflectivity_riccati",
            "reflectivity_semi_infinite", "perfect_symbol", "solve_bragg",
-           "solve_bragg_stable", "solve_surface_batch"]
+           "solve_bragg_stable", "solve_surface_batch",
+           "solve_surface_fields"]

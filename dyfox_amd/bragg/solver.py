@@ -212,6 +212,69 @@ def solve_surface_batch(seg_list, s_dev, cfg, y_values, xtal, g_vec,
     """
     scalar = np.ndim(s_dev) == 0
     s_all = np.atleast_1d(np.asarray(s_dev, np.float64))
+    shared = {}
+
+    def tables(sys_idx, ys):
+        # one field for every system: one table, reused by every call
+        key = (float(ys[0]), len(ys))
+        if shared.get("key") != key:
+            shared.clear()
+            shared["E"] = chunk_phase(seg_list, cfg, xtal, g_vec, ys)
+            shared["key"] = key
+        return shared["E"]
+
+    out, infos = _surface_solve(tables, s_all, cfg, y_values, xtal,
+                                chunk=chunk, tol=tol, restart=restart,
+                                maxiter=maxiter, verbose=verbose,
+                                accept=accept, grow=grow, tries=tries,
+                                restart_cap=restart_cap, max_depth=max_depth,
+                                per_plane=per_plane, chunk_cap=chunk_cap,
+                                warm_start=warm_start, planes=planes)
+    shared.clear()
+    if scalar:
+        return out[0], infos[0]
+    return out, infos
+
+
+def solve_surface_fields(seg_lists, s_dev, cfg, y_values, xtal, g_vec,
+                         chunk=32, tol=1e-10, restart=40, maxiter=25,
+                         verbose=False, accept=1e-7, grow=4, tries=3,
+                         restart_cap=256, max_depth=6, per_plane=6,
+                         chunk_cap=32, warm_start=None, planes="joint"):
+    """solve_surface_batch for many dislocation fields on one grid: system
+    j is the field seg_lists[j] at deviation s_dev[j] (an array of the
+    same length, or a scalar for all), each with its own e^{iH} table,
+    solved in one batched GMRES per chunk under the same policy and
+    returning the same (n, ny, Nx) surface and per-system `info` dicts.
+    (No counterpart in the reference, which solves one field per call;
+    this is how the warm-start experiment solves its instances.)"""
+    n = len(seg_lists)
+    s_all = np.broadcast_to(np.asarray(s_dev, np.float64), (n,)).copy()
+
+    def tables(sys_idx, ys):
+        E = N.DeviceArray.empty((len(sys_idx), cfg.Nz, len(ys), cfg.Nx),
+                                np.complex128)
+        for k, j in enumerate(sys_idx):
+            chunk_phase(seg_lists[j], cfg, xtal, g_vec, ys, out=E[k])
+        return E
+
+    return _surface_solve(tables, s_all, cfg, y_values, xtal, chunk=chunk,
+                          tol=tol, restart=restart, maxiter=maxiter,
+                          verbose=verbose, accept=accept, grow=grow,
+                          tries=tries, restart_cap=restart_cap,
+                          max_depth=max_depth, per_plane=per_plane,
+                          chunk_cap=chunk_cap, warm_start=warm_start,
+                          planes=planes)
+
+
+def _surface_solve(tables, s_all, cfg, y_values, xtal, *, chunk, tol,
+                   restart, maxiter, verbose, accept, grow, tries,
+                   restart_cap, max_depth, per_plane, chunk_cap, warm_start,
+                   planes):
+    """The measured-residual policy of bragg/solver.py:182-379 over a
+    batch of systems.  `tables(sys_idx, ys)` gives the e^{iH} table of
+    those systems on planes ys: one (Nz, ny, Nx) table shared by all, or
+    one (len(sys_idx), Nz, ny, Nx) stack in sys_idx order."""
     n_s = len(s_all)
     y_values = np.asarray(y_values, np.float64)
     ny = len(y_values)
@@ -233,7 +296,10 @@ def solve_surface_batch(seg_list, s_dev, cfg, y_values, xtal, g_vec,
 
     def solve_block(sys_idx, j0, j1, depth=0):
         nyc = j1 - j0
-        E = chunk_phase(seg_list, cfg, xtal, g_vec, y_values[j0:j1])
+        ys = y_values[j0:j1]
+        E = tables(list(sys_idx), ys)
+        per = E.ndim == 4
+        E_sys = list(sys_idx)           # the systems E holds, in order
         cap = max(restart, min(restart_cap, 8 * nyc))
         r = max(restart, min(cap, per_plane * nyc))
         pending = list(sys_idx)
@@ -269,16 +335,19 @@ def solve_surface_batch(seg_list, s_dev, cfg, y_values, xtal, g_vec,
                 x0[i].copy_from(v[j])
             pending = [pending[j] for j in retry]
             r = min(r * grow, cap)
+            if per:
+                E, E_sys = tables(pending, ys), list(pending)
         split = [s for s in sys_idx
                  if res[s] > accept and nyc > 1 and depth < max_depth]
         keep = [s for s in sys_idx if s not in split]
         if keep:
             whole = keep == list(sys_idx) and all(
                 V[s].ptr == first[j].ptr for j, s in enumerate(keep))
+            Ek = tables(keep, ys) if per and E_sys != keep else E
             if whole and keep == list(range(keep[0], keep[0] + len(keep))):
                 # every system kept from the first attempt, contiguous in
                 # out: the exit waves go straight into their slice
-                exit_wave(cfg, xtal, E, first,
+                exit_wave(cfg, xtal, Ek, first,
                           out=out[keep[0]:keep[0] + len(keep), j0:j1])
             else:
                 vk = first if whole else N.DeviceArray.empty(
@@ -286,7 +355,7 @@ def solve_surface_batch(seg_list, s_dev, cfg, y_values, xtal, g_vec,
                 if not whole:
                     for j, s in enumerate(keep):
                         vk[j].copy_from(V[s])
-                ew = exit_wave(cfg, xtal, E, vk)
+                ew = exit_wave(cfg, xtal, Ek, vk)
                 for j, s in enumerate(keep):
                     out[s, j0:j1] = ew[j]
             for s in keep:
@@ -307,7 +376,10 @@ def solve_surface_batch(seg_list, s_dev, cfg, y_values, xtal, g_vec,
 
     def solve_planes(j0, j1):
         nyc = j1 - j0
-        E = chunk_phase(seg_list, cfg, xtal, g_vec, y_values[j0:j1])
+        ys = y_values[j0:j1]
+        E = tables(list(range(n_s)), ys)
+        per = E.ndim == 4
+        full = True                     # E holds every system
         cap = max(restart, min(restart_cap, 8))
         r = max(restart, min(cap, per_plane))
         pending = np.arange(n_s)
@@ -338,6 +410,10 @@ def solve_surface_batch(seg_list, s_dev, cfg, y_values, xtal, g_vec,
                 x0[j].copy_from(V[s])
             pending = np.asarray(retry)
             r = min(r * grow, cap)
+            if per:
+                E, full = tables(list(pending), ys), False
+        if not full:
+            E = tables(list(range(n_s)), ys)
         exit_wave(cfg, xtal, E, V, out=out[:, j0:j1])
         unconv = (res > accept).sum(axis=1)
         worst, its = res.max(axis=1), cnt.max(axis=1)
@@ -366,8 +442,6 @@ def solve_surface_batch(seg_list, s_dev, cfg, y_values, xtal, g_vec,
               "retries": st["retries"], "splits": st["splits"],
               "unconverged_chunks": st["unconverged"],
               "converged": st["unconverged"] == 0} for st in stats]
-    if scalar:
-        return out[0], infos[0]
     return out, infos
 
 

@@ -430,3 +430,37 @@ def test_shooting_agrees_with_the_fixed_point_where_both_work(dev,
     b, info = solve_bragg_stable([], sd, cfg, diamond400, g)
     assert it["converged"] and info["residual"] < 1e-8
     assert np.abs(a - b).max() < 2e-3 * np.abs(a).max()
+
+
+@pytest.mark.parametrize("planes", ["joint", "independent"])
+def test_surface_fields_equal_one_field_per_call(dev, diamond400, g, planes):
+    """solve_surface_fields (one field per system, one batched GMRES) gives
+    each system exactly what solve_surface_batch gives that field alone,
+    integer info included, with a Krylov space small enough that some
+    systems retry and split while others do not."""
+    cfg = BraggConfig(diamond400, t_crystal=4e-6, dx=0.5e-6, half_x=4e-6)
+    ys = (np.arange(4) - 2) * cfg.dx + cfg.dx / 2
+    b = diamond400.b_mag
+    fields = [threading_segment("Edge", +1, b, float(g[2]), y0=0.3e-6,
+                                x0=-0.4e-6),
+              [],
+              threading_segment("Screw", -1, b, float(g[2]), y0=-0.2e-6,
+                                x0=0.6e-6, azimuth_deg=30.0),
+              threading_segment("Edge", -1, b, float(g[2]), x0=0.9e-6)]
+    s = deviation_from_urad(np.array([-10.0, 4.0, 0.0, 15.0]), diamond400)
+    kw = dict(restart=1, per_plane=0, maxiter=1, restart_cap=3, grow=3,
+              tries=2, planes=planes)
+    out, infos = BS.solve_surface_fields(fields, s, cfg, ys, diamond400, g,
+                                         **kw)
+    for j, segs in enumerate(fields):
+        one, info = solve_surface_batch(segs, float(s[j]), cfg, ys,
+                                        diamond400, g, **kw)
+        for k in ("iterations", "retries", "splits", "unconverged_chunks",
+                  "converged"):
+            assert info[k] == infos[j][k], (j, k, info, infos[j])
+        assert infos[j]["residual"] == pytest.approx(info["residual"],
+                                                     rel=1e-6, abs=1e-18)
+        assert _rel(out[j], one) < 1e-12
+    if planes == "joint":
+        assert any(i["splits"] for i in infos)
+        assert any(i["retries"] for i in infos)
