@@ -60,16 +60,26 @@ def smooth_thicknesses(xtal, t_range=(20e-6, 40e-6), primes=(2, 3, 5, 7)):
     return np.array(out)
 
 
-def draw_instances(n, rng, xtal, t_range=(20e-6, 40e-6), max_seg=12,
-                   urad=60.0, smooth=True):
-    """n instances: thickness (uniform over the FFT-friendly values of
-    t_range when `smooth`, else uniform in t_range), s_dev (uniform in
-    +-urad), 0..max_seg segments."""
+def thickness_grid(xtal, t_range=(20e-6, 100e-6), n=40):
+    """n thicknesses evenly spread over t_range, each snapped to the Bragg
+    grid's depth step (dz = dx tan(theta_B)); instances drawn from them
+    share grids in groups, so each group is one batched GMRES."""
+    dz = DX * xtal.tan_tB
+    nz = np.unique(np.round(np.linspace(t_range[0], t_range[1], n) / dz))
+    return nz * dz
+
+
+def draw_instances(n, rng, xtal, t_range=(20e-6, 100e-6), max_seg=12,
+                   urad=60.0, smooth=False, t_values=None):
+    """n instances: thickness uniform over `t_values` (default: over the
+    FFT-friendly values of t_range when `smooth`, else uniform in
+    t_range), s_dev uniform in +-urad, 0..max_seg segments."""
     dirs = family_110()
-    ts = smooth_thicknesses(xtal, t_range) if smooth else None
+    ts = smooth_thicknesses(xtal, t_range) if smooth else t_values
     out = []
     for _ in range(n):
-        t = ts[rng.integers(len(ts))] if smooth else rng.uniform(*t_range)
+        t = ts[rng.integers(len(ts))] if ts is not None else \
+            rng.uniform(*t_range)
         cfg = BraggConfig(xtal, t_crystal=t, dx=DX, half_x=HALF_X)
         segs = []
         for _ in range(int(rng.integers(0, max_seg + 1))):
@@ -114,10 +124,38 @@ def _groups(instances, max_sys):
 
 
 def _norms(xtal, inst):
+    """The network's two scalar inputs: s_dev in units of 60 urad, and the
+    thickness mapped from [20, 100] um to [-1, 1]."""
     from ..units import urad_from_deviation
     s = float(urad_from_deviation(inst["s"], xtal)) / 60.0
-    t = (inst["t"] - 30e-6) / 10e-6
+    t = (inst["t"] - 60e-6) / 40e-6
     return s, t
+
+
+def write_features(xtal, instances, ds, idx, max_sys=64):
+    """Network inputs of instances idx into ds.X (their e^{iH} tables are
+    built chunk by chunk; nothing is solved)."""
+    g = bragg_g_vector(xtal)
+    ys = plane_ys()
+    for grp in _groups([instances[i] for i in idx], max_sys):
+        members = [idx[k] for k in grp]
+        cfg = BraggConfig(xtal, t_crystal=instances[members[0]]["t"], dx=DX,
+                          half_x=HALF_X)
+        x0c = (cfg.Nx - W0) // 2
+        sn = np.array([_norms(xtal, instances[i])[0] for i in members],
+                      np.float32)
+        tn = np.array([_norms(xtal, instances[i])[1] for i in members],
+                      np.float32)
+        snd, tnd = N.DeviceArray.from_host(sn), N.DeviceArray.from_host(tn)
+        for c0 in range(0, NPLANES, CHUNK):
+            for j, i in enumerate(members):
+                E = chunk_phase(instances[i]["segs"], cfg, xtal, g,
+                                ys[c0:c0 + CHUNK])
+                N.call("dyf_nn_bragg_features", C.c_void_p(E.ptr), 1,
+                       cfg.Nz, CHUNK, cfg.Nx, x0c, W0, NK,
+                       C.c_void_p(snd.ptr + 4 * j), C.c_void_p(tnd.ptr + 4 * j),
+                       C.c_void_p(ds.X.ptr), H0, c0, int(i), None)
+                del E
 
 
 def solve_group(xtal, idx, instances, *, tol=1e-10, warm=None, ds_out=None,
@@ -174,11 +212,16 @@ def solve_group(xtal, idx, instances, *, tol=1e-10, warm=None, ds_out=None,
     return v_all, its, res, cfg
 
 
-def build_dataset(xtal, instances, max_sys=64, log=None):
-    """Cold-solve every instance, record iterations, features, targets."""
+def build_dataset(xtal, instances, max_sys=64, log=None, n_solve=None):
+    """Cold-solve instances [0, n_solve) (default all): iterations,
+    features, targets; the rest get their features only."""
     ds = Dataset(len(instances))
+    n_solve = len(instances) if n_solve is None else n_solve
+    if n_solve < len(instances):
+        write_features(xtal, instances, ds,
+                       list(range(n_solve, len(instances))), max_sys)
     x0c = None
-    for idx in _groups(instances, max_sys):
+    for idx in _groups(instances[:n_solve], max_sys):
         v, its, res, cfg = solve_group(xtal, idx, instances, ds_out=ds)
         x0c = (cfg.Nx - W0) // 2
         for j, i in enumerate(idx):
@@ -192,9 +235,10 @@ def build_dataset(xtal, instances, max_sys=64, log=None):
             log(f"  solved Nz={cfg.Nz} Nx={cfg.Nx} x{len(idx)}: its "
                 f"{its.min()}-{its.max()}, worst res {res.max():.1e}")
     ds.rms = np.sqrt(np.mean(ds.Y.astype(np.float64) ** 2, axis=(1, 2, 3)))
-    ds.scale = float(np.median(ds.rms)) or 1.0
-    if log:
-        q = np.percentile(ds.rms / ds.scale, [0, 10, 50, 90, 99, 100])
+    ds.scale = (float(np.median(ds.rms[:n_solve])) or 1.0) if n_solve \
+        else 1.0
+    if log and n_solve:
+        q = np.percentile(ds.rms[:n_solve] / ds.scale, [0, 10, 50, 90, 99, 100])
         top = np.argsort(ds.rms)[-5:][::-1]
         log("  target rms / median: min %.3g p10 %.3g p50 %.3g p90 %.3g "
             "p99 %.3g max %.3g" % tuple(q))
@@ -206,25 +250,48 @@ def build_dataset(xtal, instances, max_sys=64, log=None):
     return ds
 
 
+def _gather(ds, idx, batch):
+    """Samples idx of the dataset as one contiguous device batch (inputs,
+    targets)."""
+    per = H0 * W0 * IN_CH
+    tper = H0 * W0 * 2
+    xd = N.DeviceArray.zeros((batch * per,), np.uint16)
+    yd = N.DeviceArray.zeros((batch, H0, W0, 2), np.float32)
+    for j, i in enumerate(idx):
+        N.call("dyf_memcpy_d2d", C.c_void_p(xd.ptr + 2 * j * per),
+               C.c_void_p(ds.X.ptr + 2 * int(i) * per), C.c_size_t(2 * per))
+        N.call("dyf_memcpy_d2d", C.c_void_p(yd.ptr + 4 * j * tper),
+               C.c_void_p(ds.Yd.ptr + 4 * int(i) * tper),
+               C.c_size_t(4 * tper))
+    return xd, yd
+
+
 def train(ds, n_train, batch=32, epochs=20, lr=1e-3, seed=0, warmup=0.05,
-          log=None, net=None):
-    """Adam on the first n_train samples: linear warm-up over the first
-    `warmup` fraction of the steps, then cosine decay to 2% of lr.
-    Returns the trained network and the per-epoch mean losses."""
+          log=None, net=None, idx=None):
+    """Adam on the first n_train samples, or on the samples `idx` (drawn
+    into batches each epoch): linear warm-up over the first `warmup`
+    fraction of the steps, then cosine decay to 2% of lr.  Returns the
+    trained network and the per-epoch mean losses."""
     net = net or WarmStartNet(batch, seed=seed)
-    steps = n_train // batch
+    pool = np.arange(n_train) if idx is None else np.asarray(idx)
+    steps = len(pool) // batch
     rng = np.random.default_rng(seed)
     per = H0 * W0 * IN_CH
     losses = []
     total = epochs * steps
     n_warm = max(1, int(warmup * total))
     it = 0
+    contiguous = idx is None
     for ep in range(epochs):
         order = rng.permutation(steps)
+        perm = rng.permutation(pool)
         acc = 0.0
         for b in order:
-            x = ds.X[b * batch * per:(b + 1) * batch * per]
-            yd = ds.Yd[b * batch:(b + 1) * batch]
+            if contiguous:
+                x = ds.X[b * batch * per:(b + 1) * batch * per]
+                yd = ds.Yd[b * batch:(b + 1) * batch]
+            else:
+                x, yd = _gather(ds, perm[b * batch:(b + 1) * batch], batch)
             net.forward(x)
             loss = net.backward(yd)
             if it < n_warm:
@@ -253,7 +320,8 @@ def predict(net, ds, idx):
         xd = N.DeviceArray.zeros((B * per,), np.uint16)
         for j, i in enumerate(chunk):
             N.call("dyf_memcpy_d2d", C.c_void_p(xd.ptr + 2 * j * per),
-                   C.c_void_p(ds.X.ptr + 2 * i * per), C.c_size_t(2 * per))
+                   C.c_void_p(ds.X.ptr + 2 * int(i) * per),
+                   C.c_size_t(2 * per))
         o = net.forward(xd).get().reshape(B, H0, W0, 2)[:len(chunk)]
         out[j0:j0 + len(chunk)] = (o[..., 0] + 1j * o[..., 1]) * ds.scale
     return out
@@ -356,4 +424,124 @@ def evaluate(xtal, instances, ds, net, idx, max_sys=64, log=None,
     if log:
         for name in names:
             log(f"  eval {name:16s} {per[name]}")
+    return rep
+
+
+BANDS_UM = ((20, 35), (35, 50), (50, 70), (70, 100))
+
+
+def in_band(t_um, lo, hi):
+    """Thickness band membership; thicknesses are snapped to the depth
+    step, so the range's ends (20 and 100 um) land up to one step (0.23
+    um) inside or outside them and are counted in the end bands."""
+    lo_edge = lo - 0.5 if lo == BANDS_UM[0][0] else lo
+    hi_edge = hi + 0.5 if hi == BANDS_UM[-1][1] else hi
+    return (t_um >= lo_edge) & (t_um < hi_edge)
+
+
+def rescale(ds, mask):
+    """Normalise the targets by the median rms of the samples in `mask`
+    (the converged ones: an unconverged solve is no target)."""
+    ds.scale = float(np.median(ds.rms[mask])) or 1.0
+    ds.Yd = N.DeviceArray.from_host(ds.Y / ds.scale)
+
+
+def evaluate_joint(xtal, instances, ds, net, idx, max_sys=64, log=None,
+                   tol=1e-10, planes="joint"):
+    """The held-out instances idx solved with the reference's policy
+    (solve_surface_fields: with planes="joint" its 32-plane chunks,
+    retries and splits; "independent" one Krylov space per plane) cold,
+    warm from the perfect-crystal value and warm from the network
+    (perfect value plus the predicted field; inference time included).
+    Instances of one grid are one batched solve per variant, timed
+    after an untimed call that sets up the workspace.  Returns per
+    variant and per thickness band: solves/s, mean GMRES iterations,
+    retries and splits per solve, converged fraction, worst residual."""
+    from ..bragg.solver import solve_surface_fields
+    sub = [instances[i] for i in idx]
+    N.synchronize()
+    t0 = time.perf_counter()
+    pred = predict(net, ds, idx)
+    N.synchronize()
+    infer_t = time.perf_counter() - t0
+    g = bragg_g_vector(xtal)
+    ys = plane_ys()
+    names = ("cold", "perfect", "network")
+    rec = {k: {"t": np.zeros(len(idx)), "its": np.zeros(len(idx)),
+               "ret": np.zeros(len(idx)), "spl": np.zeros(len(idx)),
+               "conv": np.zeros(len(idx), bool), "res": np.zeros(len(idx))}
+           for k in names}
+    for grp in _groups(sub, max_sys):
+        cfg = BraggConfig(xtal, t_crystal=sub[grp[0]]["t"], dx=DX,
+                          half_x=HALF_X)
+        segs = [sub[k]["segs"] for k in grp]
+        s = np.array([sub[k]["s"] for k in grp])
+        x0c = (cfg.Nx - W0) // 2
+        vp = np.array([perfect_v(xtal, sub[k]["s"], cfg) for k in grp])
+        base = np.broadcast_to(vp[:, None, None],
+                               (len(grp), NPLANES, cfg.Nx)).copy()
+        net_w = base.copy()
+        net_w[:, :, x0c:x0c + W0] += pred[grp]
+        guesses = {"cold": None, "perfect": base, "network": net_w}
+        solve_surface_fields(segs, s, cfg, ys, xtal, g, tol=1.0,
+                             planes=planes)
+        for name in names:
+            N.synchronize()
+            t1 = time.perf_counter()
+            _, infos = solve_surface_fields(segs, s, cfg, ys, xtal, g,
+                                            tol=tol, planes=planes,
+                                            warm_start=guesses[name])
+            dt = time.perf_counter() - t1
+            if log:
+                log(f"    {name}: Nz={cfg.Nz} x{len(grp)} {dt:.2f} s")
+            if name == "network":
+                dt += infer_t * len(grp) / len(idx)
+            r = rec[name]
+            for j, k in enumerate(grp):
+                r["t"][k] = dt / len(grp)
+                r["its"][k] = infos[j]["iterations"]
+                r["ret"][k] = infos[j]["retries"]
+                r["spl"][k] = infos[j]["splits"]
+                r["conv"][k] = infos[j]["converged"]
+                r["res"][k] = infos[j]["residual"]
+        if log:
+            log(f"  Nz={cfg.Nz} Nx={cfg.Nx} x{len(grp)}: " + ", ".join(
+                f"{n} {rec[n]['t'][grp].sum():.2f} s its "
+                f"{rec[n]['its'][grp].mean():.0f}" for n in names))
+    t_um = np.array([x["t"] for x in sub]) * 1e6
+
+    def summary(mask):
+        out = {}
+        for name in names:
+            r = rec[name]
+            n = int(mask.sum())
+            out[name] = {"n": n,
+                         "solves_per_s": n / max(r["t"][mask].sum(), 1e-12),
+                         "iterations_mean": float(r["its"][mask].mean()),
+                         "retries_mean": float(r["ret"][mask].mean()),
+                         "splits_mean": float(r["spl"][mask].mean()),
+                         "converged_fraction": float(r["conv"][mask].mean()),
+                         "worst_residual": float(r["res"][mask].max())}
+        for name in ("perfect", "network"):
+            out[name]["speedup_vs_cold"] = (out[name]["solves_per_s"] /
+                                            out["cold"]["solves_per_s"])
+        return out
+
+    policy = ("joint 32-plane chunks, retries and splits "
+              "(bragg/solver.py:182-379)" if planes == "joint" else
+              "one Krylov space per plane (restart 40)")
+    rep = {"all": summary(np.ones(len(idx), bool)), "bands": {},
+           "inference_s": infer_t, "policy": f"{policy}, tol {tol:g}"}
+    for lo, hi in BANDS_UM:
+        m = in_band(t_um, lo, hi)
+        if m.any():
+            rep["bands"][f"{lo}-{hi}um"] = summary(m)
+    if log:
+        for k, v in [("all", rep["all"])] + list(rep["bands"].items()):
+            log(f"  eval {k}: " + "; ".join(
+                f"{n} {v[n]['solves_per_s']:.2f}/s its "
+                f"{v[n]['iterations_mean']:.0f} conv "
+                f"{v[n]['converged_fraction']:.2f} retries "
+                f"{v[n]['retries_mean']:.2f} splits {v[n]['splits_mean']:.2f}"
+                for n in names))
     return rep

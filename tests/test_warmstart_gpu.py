@@ -108,8 +108,9 @@ def test_training_reduces_the_loss(dev, rng):
 def test_warm_start_converges_to_the_cold_answer(dev, diamond400):
     from dyfox_amd.warmstart import data
     rng = np.random.default_rng(5)
-    inst = data.draw_instances(40, rng, diamond400, t_range=(20e-6, 22.5e-6),
-                               max_seg=4)
+    inst = data.draw_instances(
+        40, rng, diamond400, max_seg=4,
+        t_values=data.thickness_grid(diamond400, (20e-6, 22.5e-6), 2))
     ds = data.build_dataset(diamond400, inst, max_sys=40)
     assert (ds.resid <= 1e-10).all()
     model, losses = data.train(ds, 32, batch=8, epochs=3)
@@ -119,3 +120,28 @@ def test_warm_start_converges_to_the_cold_answer(dev, diamond400):
     assert rep["warm_worst_residual"] <= 1e-10
     assert rep["cold_worst_residual"] <= 1e-10
     assert rep["max_abs_diff_warm_vs_cold_v"] < 1e-8
+
+
+def test_joint_policy_evaluation_by_band(dev, diamond400):
+    """The held-out evaluation path: features written without solving,
+    the reference's joint policy cold and warm, the same converged
+    answer, and the report's per-band structure."""
+    from dyfox_amd.warmstart import data
+    rng = np.random.default_rng(7)
+    ts = data.thickness_grid(diamond400, (20e-6, 37e-6), 2)
+    inst = (data.draw_instances(16, rng, diamond400, max_seg=3,
+                                t_values=ts[:1]) +
+            data.draw_instances(6, rng, diamond400, max_seg=3, t_values=ts))
+    ds = data.build_dataset(diamond400, inst, max_sys=16, n_solve=16)
+    assert np.abs(ds.Y[16:]).max() == 0.0          # held out: features only
+    x = ds.X.get().reshape(len(inst), -1)
+    assert (np.abs(x[16:].astype(np.int64)).sum(axis=1) > 0).all()
+    model, _ = data.train(ds, 0, batch=8, epochs=1, idx=np.arange(16))
+    rep = data.evaluate_joint(diamond400, inst, ds, model,
+                              list(range(16, 22)))
+    print(rep)
+    for name in ("cold", "perfect", "network"):
+        assert rep["all"][name]["n"] == 6
+        assert rep["all"][name]["converged_fraction"] == 1.0
+        assert rep["all"][name]["worst_residual"] <= 1e-10
+    assert set(rep["bands"]) <= {"20-35um", "35-50um"}
