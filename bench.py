@@ -61,6 +61,10 @@ def parse():
                     help="run only the rocking-scan solves (FP64 then "
                          "mixed; one warm-up and five timed each), for a "
                          "kernel trace, and exit")
+    ap.add_argument("--probe-matvec", action="store_true",
+                    help="run only the 20 um Bragg matvec timing (20 "
+                         "launches of the full 1312-row march), for "
+                         "counter passes, and exit")
     ap.add_argument("--probe-gemm", action="store_true",
                     help="time the warm-start network's GEMMs (forward, "
                          "data and weight gradients of every 3x3 layer at "
@@ -616,6 +620,17 @@ def main():
         return probe_laue()
     if args.probe_gemm:
         return probe_gemm()
+    if args.probe_matvec:
+        from dyfox_amd import _native as N
+        from dyfox_amd.backend import device
+        from dyfox_amd.crystal import DIAMOND, CrystalParams
+        device()
+        xtal = CrystalParams.from_reflection(DIAMOND, (4, 0, 0), 17.0,
+                                             f_prime=0.003, f_dprime=0.002)
+        r = time_bragg_matvec(N, xtal)
+        print(f"probe-matvec done: {r['avg_launch_ms'] * 1e3:.1f} us per "
+              f"launch", flush=True)
+        return None
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: there is no multi-GPU path here "
               "(the work would shard over rows x s_dev with no "
