@@ -368,7 +368,7 @@ using gptr = const T __attribute__((address_space(1))) *;
 template <typename T>
 using gptr_w = T __attribute__((address_space(1))) *;
 
-template <int C, int PREC>
+template <int C, int PREC, bool ROLL = false>
 __device__ __forceinline__ void fast_row_unrolled(const FastArgs &a, int r,
                                                   int ie) {
     using S = StateT<PREC>;
@@ -390,12 +390,28 @@ __device__ __forceinline__ void fast_row_unrolled(const FastArgs &a, int r,
     }
     gptr<cx<T>> E = (gptr<cx<T>>)a.phase + (size_t)r * a.nsteps * kSlice + lane;
 
-    auto step = [&](int k) {
-        cx<T> Ec[C];
-        gptr<cx<T>> Ek = E + (size_t)k * kSlice;
+    // ROLL: the e^{iH} slice is prefetched on a rolling basis, as in
+    // fast_row (column i's value for step k+1 is loaded as soon as column
+    // i of step k has used it); otherwise each step loads its slice first.
+    cx<T> Ec[C];
+    if constexpr (ROLL) {
 #pragma unroll
-        for (int i = 0; i < C; ++i)
-            Ec[i] = cx<T>{Ek[i * kWave].r, Ek[i * kWave].i};
+        for (int i = 0; i < C; ++i) {
+            Ec[i] = cx<T>{E[i * kWave].r, E[i * kWave].i};
+            if constexpr (sizeof(T) == 4) __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+
+    auto step = [&](int k) {
+        gptr<cx<T>> Ek = E + (size_t)k * kSlice;
+        if constexpr (ROLL) {
+            const int kn = k + 1 < a.nsteps ? k + 1 : k;
+            Ek = E + (size_t)kn * kSlice;
+        } else {
+#pragma unroll
+            for (int i = 0; i < C; ++i)
+                Ec[i] = cx<T>{Ek[i * kWave].r, Ek[i * kWave].i};
+        }
         T cr, ci, br, bi;
         const size_t ci_ = (size_t)__builtin_amdgcn_readfirstlane(ie) * a.nsteps + k;
         if constexpr (sizeof(T) == 8) {
@@ -427,6 +443,11 @@ __device__ __forceinline__ void fast_row_unrolled(const FastArgs &a, int r,
             const T ti = fmaT(er, yi, ei * yr);
             const T ur = fmaT(er, xr, ei * xi);     // u = e^{-iH} x
             const T ui = fmaT(er, xi, -ei * xr);
+            if constexpr (ROLL) {
+                asm volatile("" ::: "memory");
+                Ec[i] = cx<T>{Ek[i * kWave].r, Ek[i * kWave].i};
+                if constexpr (sizeof(T) == 4) __builtin_amdgcn_sched_barrier(0);
+            }
             if constexpr (!COMP) {
                 nXr[i] = fmaT(cr, tr, fmaT(-ci, ti, xr));
                 nXi[i] = fmaT(cr, ti, fmaT(ci, tr, xi));
@@ -463,7 +484,10 @@ __device__ __forceinline__ void fast_row_unrolled(const FastArgs &a, int r,
     int k = 0;
     for (; k + C <= a.nsteps; k += C) {
 #pragma unroll
-        for (int kk = 0; kk < C; ++kk) step(k + kk);
+        for (int kk = 0; kk < C; ++kk) {
+            step(k + kk);
+            if constexpr (ROLL) __builtin_amdgcn_sched_barrier(0);
+        }
     }
     for (; k < a.nsteps; ++k) step(k);
 
@@ -493,6 +517,22 @@ __global__ __launch_bounds__(64) void laue_fast_kernel(FastArgs a) {
     if (L >= a.total) return;
     const int r = L / a.n_e, ie = L - r * a.n_e;
     fast_row<C, MODE, PREC>(a, r, ie);
+}
+
+// The grid-mode march with FP32 state (FP32 and MIXED): the step loop
+// unrolled by C with the rolling prefetch.  Each step moves every column
+// one register over, so after C steps the register assignment is back
+// where it started and the loop-carried state needs no copies (C = 17
+// mixed: 497 VALU instructions per step instead of 553).  With FP64 state
+// the register allocator does not find that assignment within two waves
+// per SIMD and spills, so FP64 keeps the rolled fast_row.
+template <int C, int PREC>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+void laue_fast_unrolled_kernel(FastArgs a) {
+    const int L = logical_block(a.per_xcd);
+    if (L >= a.total) return;
+    const int r = L / a.n_e, ie = L - r * a.n_e;
+    fast_row_unrolled<C, PREC, true>(a, r, ie);
 }
 
 // ---- general path: direct form, any shift, LDS rows ------------------
@@ -746,7 +786,10 @@ template <int MODE, int PREC, int... Cs>
 struct FastTable {
     static FastFn get(int C) {
         FastFn f = nullptr;
-        ((C == Cs ? (f = laue_fast_kernel<Cs, MODE, PREC>, 0) : 0), ...);
+        if constexpr (MODE == DYF_MODE_GRID && PREC != DYF_FP64)
+            ((C == Cs ? (f = laue_fast_unrolled_kernel<Cs, PREC>, 0) : 0), ...);
+        else
+            ((C == Cs ? (f = laue_fast_kernel<Cs, MODE, PREC>, 0) : 0), ...);
         return f;
     }
 };
