@@ -695,10 +695,10 @@ def main():
           f"planes {bragg['edge_line_independent_planes']['solves_per_s']:.1f})",
           file=sys.stderr, flush=True)
     bragg_roof = time_bragg_matvec(N, xtal)
-    warm = time_warm_start()
-    ws = warm["independent_planes"]["all"]
-    print(f"[bench] learned warm start, {warm['n_test']} instances 20-100 "
-          f"um: cold {ws['cold']['solves_per_s']:.1f} solves/s, warm "
+    warm_start = time_warm_start()
+    ws = warm_start["independent_planes"]["all"]
+    print(f"[bench] learned warm start, {warm_start['n_test']} instances "
+          f"20-100 um: cold {ws['cold']['solves_per_s']:.1f} solves/s, warm "
           f"{ws['network']['solves_per_s']:.1f} "
           f"({ws['warm_over_cold']:.2f}x)", file=sys.stderr, flush=True)
 
@@ -748,7 +748,7 @@ def main():
         "imaging_rocking_scan_273um": imaging,
         "illumination_rocking_scan_273um": illum,
         "laue_ensemble_256": None if args.no_ensemble else time_ensembles(),
-        "bragg_warm_start_cfg5": warm,
+        "bragg_warm_start_cfg5": warm_start,
         "roofline": roof,
         "bragg_roofline": bragg_roof,
     }
