@@ -598,8 +598,8 @@ def probe_gemm(batch=32, repeats=10):
 
 
 def probe_bragg():
-    """The 20 um Bragg surface batch with one edge line, independent
-    planes: one warm-up batch, five timed."""
+    """The 20 um Bragg surface batch with one edge line on the
+    reference's joint 32-plane chunk: one warm-up batch, five timed."""
     from dyfox_amd.backend import device
     from dyfox_amd.crystal import DIAMOND, CrystalParams
     device()

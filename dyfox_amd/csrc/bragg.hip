@@ -829,23 +829,6 @@ __device__ int arnoldi_finish_sys(int s, int restart, int col, cd *H, cd *G,
     return 1;
 }
 
-__global__ void arnoldi_finish_kernel(int nsys, int restart, int col, cd *H,
-                                      cd *G, cd *S, const cd *h1st,
-                                      const cd *h2nd, const cd *nrm_w,
-                                      const cd *nrm0, SysState *st,
-                                      double *inv, int *active, double eps,
-                                      int *still) {
-    const int s = blockIdx.x * 64 + threadIdx.x;
-    if (s >= nsys || !active[s]) return;
-    cd *h = H + (size_t)s * restart * (restart + 1) + (size_t)col * (restart + 1);
-    arnoldi_finish_sys(s, restart, col, H, G, S,
-                       h1st + (size_t)s * (restart + 1),
-                       h2nd + (size_t)s * (restart + 1),
-                       G + (size_t)s * restart * 2, h,
-                       S[(size_t)s * (restart + 1) + col], nrm_w[s].r,
-                       nrm0[s].r, st, inv, active, eps, still);
-}
-
 // One Arnoldi step after w = M A v_col for systems of N <= kCgsMaxN
 // unknowns, one workgroup per system with w in registers: CGS2 (||w||^2,
 // then twice h = V^H w over the kc = col + 1 basis vectors and w -= V h,
@@ -1044,6 +1027,283 @@ static step_fn step_kernel(size_t N) {
     return tab[(N + T - 1) / T - 1];
 }
 
+// ---- Arnoldi step for large systems: CGS2 over slices -----------------
+// Systems too large for one workgroup (the joint chunk: N = planes x Nx,
+// 16,640 on a 32-plane, 520-column chunk) take one workgroup per (slice of
+// kSplitLen elements, system), each holding its slice of w in registers.
+// A pass that needs a whole-system sum writes one partial per slice; the
+// next kernel sums that system's partials in slice order (deterministic,
+// and every workgroup of the system gets the same value) before it uses
+// them:
+//   cgs_dots_kernel               h1 = V^H w and ||w||^2   -> partials A
+//   cgs_update_dots_kernel        w -= V h1, h2 = V^H w    -> partials B
+//   cgs_update_norm_kernel        w -= V h2, ||w||^2       -> partials C
+//   arnoldi_finish_scale_kernel   the norms, v_{col+1} = w / ||w||, and in
+//                                 slice 0 the Hessenberg/Givens update
+// four launches where the batched dot, combine, finish and scale kernels
+// took nine, and w crosses memory five times instead of ten.  The same CGS2 as arnoldi_step_kernel; only the order
+// of the floating-point sums differs.
+constexpr int kSplitThreads = 256;
+constexpr int kSplitWaves = kSplitThreads / kWave;
+constexpr int kSplitPer = 4;
+constexpr int kSplitLen = kSplitThreads * kSplitPer;
+
+static int split_slices(size_t N) {
+    return (int)((N + kSplitLen - 1) / kSplitLen);
+}
+
+// This lane's elements of slice `b`: clamped in-range indices (loads carry
+// no branch), and x = 0 past N.
+__device__ __forceinline__ void split_load(const cd *__restrict__ src,
+                                           size_t N, int b, int tid,
+                                           size_t (&idx)[kSplitPer],
+                                           bool (&ok)[kSplitPer],
+                                           cd (&x)[kSplitPer]) {
+#pragma unroll
+    for (int m = 0; m < kSplitPer; ++m) {
+        const size_t i = (size_t)b * kSplitLen + tid + (size_t)m * kSplitThreads;
+        ok[m] = i < N;
+        idx[m] = ok[m] ? i : N - 1;
+        const cd v = src[idx[m]];
+        x[m] = ok[m] ? v : cd{0.0, 0.0};
+    }
+}
+
+// red[wv * rs + 2q], red[wv * rs + 2q + 1] = this wavefront's share of
+// conj(V_q) . x for q < kc (x is zero past N); two vectors' loads in flight
+__device__ __forceinline__ void split_dots(const cd *__restrict__ Vs, size_t N,
+                                           int kc,
+                                           const size_t (&idx)[kSplitPer],
+                                           const cd (&x)[kSplitPer],
+                                           double *red, int rs, int wv,
+                                           int lane) {
+    for (int q = 0; q < kc; q += 2) {
+        const bool two = q + 1 < kc;
+        const cd *v0 = Vs + (size_t)q * N;
+        const cd *v1 = Vs + (size_t)(two ? q + 1 : q) * N;
+        cd a[kSplitPer], b[kSplitPer];
+#pragma unroll
+        for (int m = 0; m < kSplitPer; ++m) a[m] = v0[idx[m]], b[m] = v1[idx[m]];
+        double sr = 0.0, si = 0.0, tr = 0.0, ti = 0.0;
+#pragma unroll
+        for (int m = 0; m < kSplitPer; ++m) {
+            sr += a[m].r * x[m].r + a[m].i * x[m].i;
+            si += a[m].r * x[m].i - a[m].i * x[m].r;
+            tr += b[m].r * x[m].r + b[m].i * x[m].i;
+            ti += b[m].r * x[m].i - b[m].i * x[m].r;
+        }
+        sr = wave_sum(sr), si = wave_sum(si);
+        tr = wave_sum(tr), ti = wave_sum(ti);
+        if (lane == 0) {
+            red[wv * rs + 2 * q] = sr, red[wv * rs + 2 * q + 1] = si;
+            if (two) red[wv * rs + 2 * q + 2] = tr, red[wv * rs + 2 * q + 3] = ti;
+        }
+    }
+}
+
+// x -= sum_q coef_q V_q over q < kc (coef: 2 kc doubles, re/im)
+__device__ __forceinline__ void split_update(const cd *__restrict__ Vs,
+                                             size_t N, int kc,
+                                             const double *coef,
+                                             const size_t (&idx)[kSplitPer],
+                                             const bool (&ok)[kSplitPer],
+                                             cd (&x)[kSplitPer]) {
+    double ar[kSplitPer], ai[kSplitPer];
+#pragma unroll
+    for (int m = 0; m < kSplitPer; ++m) ar[m] = 0.0, ai[m] = 0.0;
+    for (int q = 0; q < kc; q += 2) {
+        const bool two = q + 1 < kc;
+        const double c0r = coef[2 * q], c0i = coef[2 * q + 1];
+        const double c1r = two ? coef[2 * q + 2] : 0.0;
+        const double c1i = two ? coef[2 * q + 3] : 0.0;
+        const cd *v0 = Vs + (size_t)q * N;
+        const cd *v1 = Vs + (size_t)(two ? q + 1 : q) * N;
+        cd a[kSplitPer], b[kSplitPer];
+#pragma unroll
+        for (int m = 0; m < kSplitPer; ++m) a[m] = v0[idx[m]], b[m] = v1[idx[m]];
+#pragma unroll
+        for (int m = 0; m < kSplitPer; ++m) {
+            ar[m] += c0r * a[m].r - c0i * a[m].i;
+            ai[m] += c0r * a[m].i + c0i * a[m].r;
+            ar[m] += c1r * b[m].r - c1i * b[m].i;
+            ai[m] += c1r * b[m].i + c1i * b[m].r;
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < kSplitPer; ++m) {
+        x[m].r = ok[m] ? x[m].r - ar[m] : 0.0;
+        x[m].i = ok[m] ? x[m].i - ai[m] : 0.0;
+    }
+}
+
+// coef[2q], coef[2q+1] = sum over the B slices of part[q * B + b], q < kc,
+// in slice order; slice 0 also stores the sums in h when given (this
+// system's first-pass projections, read by the finish)
+__device__ __forceinline__ void split_sum_parts(const cd *__restrict__ part,
+                                                int B, int kc, double *coef,
+                                                cd *h, int b, int tid) {
+    for (int q = tid; q < kc; q += kSplitThreads) {
+        const cd *p = part + (size_t)q * B;
+        double r = 0.0, i = 0.0;
+        for (int u = 0; u < B; ++u) r += p[u].r, i += p[u].i;
+        coef[2 * q] = r, coef[2 * q + 1] = i;
+        if (h && b == 0) h[q] = cd{r, i};
+    }
+}
+
+// Partials A: pA[(s (R+2) + q) B + b] = slice b's conj(V_q) . w (q < kc),
+// and at q = R + 1 its ||w||^2.
+__global__ __launch_bounds__(kSplitThreads) void cgs_dots_kernel(
+    const cd *__restrict__ V, size_t v_sys, size_t N, int R, int kc,
+    const cd *__restrict__ w, cd *__restrict__ pA, int B,
+    const int *__restrict__ active) {
+    extern __shared__ double red[];            // [waves][2 (kc + 1)]
+    const int b = blockIdx.x, s = blockIdx.y;
+    if (!active[s]) return;
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int rs = 2 * (kc + 1);
+    size_t idx[kSplitPer];
+    bool ok[kSplitPer];
+    cd x[kSplitPer];
+    split_load(w + (size_t)s * N, N, b, tid, idx, ok, x);
+    double nn = 0.0;
+#pragma unroll
+    for (int m = 0; m < kSplitPer; ++m) nn += x[m].r * x[m].r + x[m].i * x[m].i;
+    nn = wave_sum(nn);
+    if (lane == 0) red[wv * rs + 2 * kc] = nn, red[wv * rs + 2 * kc + 1] = 0.0;
+    split_dots(V + (size_t)s * v_sys, N, kc, idx, x, red, rs, wv, lane);
+    __syncthreads();
+    for (int j = tid; j < rs; j += kSplitThreads) {
+        double t = red[j];
+        for (int u = 1; u < kSplitWaves; ++u) t += red[u * rs + j];
+        const int q = j >> 1 < kc ? j >> 1 : R + 1;
+        ((double *)&pA[((size_t)s * (R + 2) + q) * B + b])[j & 1] = t;
+    }
+}
+
+// Sums partials A into h1 (its column stored in h1 by slice 0), w -= V h1
+// on the slice, then partials B: pB[(s (R+1) + q) B + b] = conj(V_q) . w.
+__global__ __launch_bounds__(kSplitThreads) void cgs_update_dots_kernel(
+    const cd *__restrict__ V, size_t v_sys, size_t N, int R, int kc,
+    cd *__restrict__ w, const cd *__restrict__ pA, cd *__restrict__ pB,
+    cd *__restrict__ h1, int B, const int *__restrict__ active) {
+    extern __shared__ double red[];            // [waves][2 kc], coef [2 kc]
+    const int b = blockIdx.x, s = blockIdx.y;
+    if (!active[s]) return;
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int rs = 2 * kc;
+    double *coef = red + kSplitWaves * rs;
+    split_sum_parts(pA + (size_t)s * (R + 2) * B, B, kc, coef,
+                    h1 + (size_t)s * (R + 1), b, tid);
+    size_t idx[kSplitPer];
+    bool ok[kSplitPer];
+    cd x[kSplitPer];
+    cd *ws = w + (size_t)s * N;
+    split_load(ws, N, b, tid, idx, ok, x);
+    __syncthreads();
+    const cd *Vs = V + (size_t)s * v_sys;
+    split_update(Vs, N, kc, coef, idx, ok, x);
+#pragma unroll
+    for (int m = 0; m < kSplitPer; ++m)
+        if (ok[m]) ws[idx[m]] = x[m];
+    split_dots(Vs, N, kc, idx, x, red, rs, wv, lane);
+    __syncthreads();
+    for (int j = tid; j < rs; j += kSplitThreads) {
+        double t = red[j];
+        for (int u = 1; u < kSplitWaves; ++u) t += red[u * rs + j];
+        ((double *)&pB[((size_t)s * (R + 1) + (j >> 1)) * B + b])[j & 1] = t;
+    }
+}
+
+// Sums partials B into h2 (stored by slice 0 for the finish), w -= V h2
+// on the slice, and the slice's ||w||^2 into partials C.
+__global__ __launch_bounds__(kSplitThreads) void cgs_update_norm_kernel(
+    const cd *__restrict__ V, size_t v_sys, size_t N, int R, int kc,
+    cd *__restrict__ w, const cd *__restrict__ pB, cd *__restrict__ h2,
+    cd *__restrict__ pC, int B, const int *__restrict__ active) {
+    extern __shared__ double red[];            // h2 [2 kc], wave sums
+    const int b = blockIdx.x, s = blockIdx.y;
+    if (!active[s]) return;
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    double *coef = red;
+    double *wsum = red + 2 * kc;
+    split_sum_parts(pB + (size_t)s * (R + 1) * B, B, kc, coef,
+                    h2 + (size_t)s * (R + 1), b, tid);
+    size_t idx[kSplitPer];
+    bool ok[kSplitPer];
+    cd x[kSplitPer];
+    cd *ws = w + (size_t)s * N;
+    split_load(ws, N, b, tid, idx, ok, x);
+    __syncthreads();
+    split_update(V + (size_t)s * v_sys, N, kc, coef, idx, ok, x);
+    double nn = 0.0;
+#pragma unroll
+    for (int m = 0; m < kSplitPer; ++m) {
+        if (ok[m]) ws[idx[m]] = x[m];
+        nn += x[m].r * x[m].r + x[m].i * x[m].i;
+    }
+    nn = wave_sum(nn);
+    if (lane == 0) wsum[wv] = nn;
+    __syncthreads();
+    if (tid == 0) {
+        double t = wsum[0];
+        for (int u = 1; u < kSplitWaves; ++u) t += wsum[u];
+        pC[(size_t)s * B + b] = cd{t, 0.0};
+    }
+}
+
+// ||w||^2 before (partials A) and after (partials C) CGS2, summed in slice
+// order by every workgroup of the system, then v_{col+1} = w / ||w|| on
+// the slice (zero on breakdown, as arnoldi_finish_sys sets 1/||w||).
+// Slice 0 also runs the Hessenberg/Givens update of arnoldi_finish_sys on
+// LDS copies of the column and the stored rotations.  When that update
+// stops the system, the other slices may or may not see the cleared
+// active flag; a stopped system never reads v_{col+1}, so either is fine.
+__global__ __launch_bounds__(kSplitThreads) void arnoldi_finish_scale_kernel(
+    cd *__restrict__ V, size_t v_sys, size_t N, int R, int col,
+    const cd *__restrict__ w, const cd *__restrict__ pA,
+    const cd *__restrict__ pC, const cd *__restrict__ h1,
+    const cd *__restrict__ h2, int B, cd *H, cd *G, cd *S, SysState *st,
+    double *inv, int *active, double eps, int *still) {
+    // LDS (doubles): partials [2 B], then slice 0's h1 / working column
+    // [2 (kc + 1)], h2 [2 kc] and rotations [4 col]
+    extern __shared__ double red[];
+    const int b = blockIdx.x, s = blockIdx.y;
+    if (!active[s]) return;
+    const int tid = threadIdx.x, kc = col + 1;
+    double *pn = red;
+    cd *h1l = (cd *)(red + 2 * B);
+    cd *h2l = h1l + (kc + 1);
+    cd *gl = h2l + kc;
+    const cd *pn0 = pA + ((size_t)s * (R + 2) + R + 1) * B;
+    for (int u = tid; u < B; u += kSplitThreads)
+        pn[u] = pn0[u].r, pn[B + u] = pC[(size_t)s * B + u].r;
+    if (b == 0) {
+        for (int t = tid; t < kc; t += kSplitThreads)
+            h1l[t] = h1[(size_t)s * (R + 1) + t],
+            h2l[t] = h2[(size_t)s * (R + 1) + t];
+        for (int t = tid; t < 2 * col; t += kSplitThreads)
+            gl[t] = G[(size_t)s * R * 2 + t];
+    }
+    __syncthreads();
+    double n0 = 0.0, nw = 0.0;
+    for (int u = 0; u < B; ++u) n0 += pn[u], nw += pn[B + u];
+    // 1/||w|| exactly as arnoldi_finish_sys forms it
+    const double hn = sqrt(nw);
+    const double sc = hn <= eps * sqrt(n0) ? 0.0 : 1.0 / hn;
+    if (b == 0 && tid == 0)
+        arnoldi_finish_sys(s, R, col, H, G, S, h1l, h2l, gl, h1l,
+                           S[(size_t)s * (R + 1) + col], nw, n0, st, inv,
+                           active, eps, still);
+    const cd *ws = w + (size_t)s * N;
+    cd *vn = V + (size_t)s * v_sys + (size_t)kc * N;
+#pragma unroll
+    for (int m = 0; m < kSplitPer; ++m) {
+        const size_t i = (size_t)b * kSplitLen + tid + (size_t)m * kSplitThreads;
+        if (i < N) vn[i] = cscale(ws[i], sc);
+    }
+}
+
 // Start of a restart cycle: inv_s = 1/||M r_s||, S_s = [||M r_s||, 0, ...]
 // (the first column of the Arnoldi basis is M r_s * inv_s).
 __global__ void cycle_start_kernel(int nsys, int R, const cd *nrm,
@@ -1163,7 +1423,7 @@ __global__ void exit_wave_kernel(const cd *__restrict__ v,
 // ---- workspace layout --------------------------------------------------
 struct Work {
     cd *V, *b, *r, *w, *av, *H, *G, *S, *Y, *h1, *h2, *nrm, *nrm0, *pinv,
-        *part;
+        *part, *pA, *pB, *pC;
     double *inv;
     SysState *st;
     int *active, *still;
@@ -1205,6 +1465,12 @@ static Work carve(void *base, int nsys, size_t N, int restart, int Nx) {
     w.nrm0 = (cd *)take(sizeof(cd) * nsys);
     w.pinv = (cd *)take(sizeof(cd) * nsys * symbol_elems(Nx));
     w.part = (cd *)take(sizeof(cd) * nsys * (R + 2) * P);
+    // partials of the sliced Arnoldi step (systems too large for one
+    // workgroup); sized for every N, since the path is chosen per solve
+    const size_t B = split_slices(N);
+    w.pA = (cd *)take(sizeof(cd) * nsys * (R + 2) * B);
+    w.pB = (cd *)take(sizeof(cd) * nsys * (R + 1) * B);
+    w.pC = (cd *)take(sizeof(cd) * nsys * B);
     w.inv = (double *)take(sizeof(double) * nsys);
     w.st = (SysState *)take(sizeof(SysState) * nsys);
     w.active = (int *)take(sizeof(int) * nsys);
@@ -2056,6 +2322,16 @@ extern "C" int dyf_bragg_solve(const dyf_bragg_args *a, const double *s_dev,
         N <= (size_t)kCgsMaxN && R < step_threads(N) &&
         sizeof(double) * (2 * (step_threads(N) / 64) + 8) * (size_t)R <=
             64 * 1024;
+    // the sliced step's LDS at the last column within 64 KB: the wave
+    // sums of cgs_update_dots_kernel, the partials, column and rotations
+    // of arnoldi_finish_scale_kernel
+    if (!fused_step &&
+        sizeof(double) * std::max<size_t>((kSplitWaves + 1) * 2 * (size_t)R,
+                                          2 * (size_t)split_slices(N) +
+                                              8 * (size_t)R + 2) >
+            64 * 1024)
+        return fail(DYF_EINVAL, "restart %d is too large for the Arnoldi "
+                    "step of a system of %zu unknowns", R, N);
 
     for (int cycle = 0; cycle < opt->maxiter; ++cycle) {
         // v0 = M r / ||M r||, S = [||M r||, 0, ...]
@@ -2087,31 +2363,35 @@ extern "C" int dyf_bragg_solve(const dyf_bragg_args *a, const double *s_dev,
                     W.st, W.inv, W.active, eps, W.still + col);
                 DYF_LAUNCHED("arnoldi_step_kernel");
             } else {
-                rc = dots(W.w, N, 0, 1, W.w, N, N, W.nrm0, 1, W.part, nsys,
-                          W.active, stream);
-                if (rc) return rc;
-                for (int pass = 0; pass < 2; ++pass) {
-                    cd *hp = pass == 0 ? W.h1 : W.h2;
-                    rc = dots(W.V, Vsys, N, col + 1, W.w, N, N, hp, R + 1,
-                              W.part, nsys, W.active, stream);
-                    if (rc) return rc;
-                    combine_kernel<<<vgrid, 256, 0, stream>>>(
-                        W.V, Vsys, N, col + 1, nullptr, hp, R + 1, -1.0, W.w,
-                        N, N, W.active);
-                    DYF_LAUNCHED("combine_kernel");
-                }
-                rc = dots(W.w, N, 0, 1, W.w, N, N, W.nrm, 1, W.part, nsys,
-                          W.active, stream);
-                if (rc) return rc;
-                arnoldi_finish_kernel<<<(nsys + 63) / 64, 64, 0, stream>>>(
-                    nsys, R, col, W.H, W.G, W.S, W.h1, W.h2, W.nrm, W.nrm0,
-                    W.st, W.inv, W.active, eps, W.still + col);
-                DYF_LAUNCHED("arnoldi_finish_kernel");
-                // v_{col+1} = w / h1 for the systems still iterating
-                scale_copy_kernel<<<vgrid, 256, 0, stream>>>(
-                    W.w, N, W.V + (size_t)(col + 1) * N, Vsys, W.inv, N,
-                    W.active);
-                DYF_LAUNCHED("scale_copy_kernel");
+                // CGS2 over slices of kSplitLen elements (three passes
+                // over the basis), then the finish on the partial sums
+                const int kc = col + 1, B = split_slices(N);
+                const dim3 sg((unsigned)B, (unsigned)nsys);
+                cgs_dots_kernel<<<sg, kSplitThreads,
+                                  sizeof(double) * kSplitWaves * 2 * (kc + 1),
+                                  stream>>>(W.V, Vsys, N, R, kc, W.w, W.pA, B,
+                                            W.active);
+                DYF_LAUNCHED("cgs_dots_kernel");
+                cgs_update_dots_kernel<<<sg, kSplitThreads,
+                                         sizeof(double) * (kSplitWaves + 1) *
+                                             2 * kc,
+                                         stream>>>(W.V, Vsys, N, R, kc, W.w,
+                                                   W.pA, W.pB, W.h1, B,
+                                                   W.active);
+                DYF_LAUNCHED("cgs_update_dots_kernel");
+                cgs_update_norm_kernel<<<sg, kSplitThreads,
+                                         sizeof(double) * (2 * kc + kSplitWaves),
+                                         stream>>>(W.V, Vsys, N, R, kc, W.w,
+                                                   W.pB, W.h2, W.pC, B,
+                                                   W.active);
+                DYF_LAUNCHED("cgs_update_norm_kernel");
+                arnoldi_finish_scale_kernel<<<
+                    sg, kSplitThreads,
+                    sizeof(double) * (2 * B + 4 * kc + 2 + 4 * col),
+                    stream>>>(W.V, Vsys, N, R, col, W.w, W.pA, W.pC, W.h1,
+                              W.h2, B, W.H, W.G, W.S, W.st, W.inv, W.active,
+                              eps, W.still + col);
+                DYF_LAUNCHED("arnoldi_finish_scale_kernel");
             }
             DYF_HIP(hipMemcpyAsync(&still_h[col & 1], W.still + col, sizeof(int),
                                    hipMemcpyDeviceToHost, stream));
