@@ -11,6 +11,7 @@ DYF_EKEY -> KeyError, DYF_EHIP/DYF_ENODEV -> RuntimeError.
 """
 
 import ctypes as C
+import math
 import os
 import threading
 
@@ -450,8 +451,7 @@ class DeviceArray:
         self.shape = tuple(int(s) for s in np.atleast_1d(shape)) \
             if not isinstance(shape, tuple) else tuple(int(s) for s in shape)
         self.dtype = np.dtype(dtype)
-        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * \
-            self.dtype.itemsize
+        self.nbytes = math.prod(self.shape) * self.dtype.itemsize
         self._block = block if block is not None else _Block(self.nbytes)
         self._offset = int(offset)
 
@@ -483,7 +483,7 @@ class DeviceArray:
 
     @property
     def size(self):
-        return int(np.prod(self.shape, dtype=np.int64))
+        return math.prod(self.shape)
 
     @property
     def ndim(self):
@@ -535,7 +535,7 @@ class DeviceArray:
     def __getitem__(self, key):
         if not self.shape:
             raise IndexError("0-d device array")
-        row = int(np.prod(self.shape[1:], dtype=np.int64)) * self.dtype.itemsize
+        row = math.prod(self.shape[1:]) * self.dtype.itemsize
         if isinstance(key, (int, np.integer)):
             k = int(key)
             if k < 0:

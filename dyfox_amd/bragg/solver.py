@@ -307,17 +307,19 @@ def _surface_solve(tables, s_all, cfg, y_values, xtal, *, chunk, tol,
             np.ascontiguousarray(warm[pending, j0:j1]))
         res = {}
         cnt = {s: 0 for s in pending}
-        V = {}
+        V = {}                       # system -> (solutions array, its row)
         first = None                 # the first attempt's solutions
         for _ in range(tries):
             v, info = gmres_batch(cfg, xtal, s_all[pending], E, inc, tol=tol,
                                   restart=r, maxiter=maxiter, x0=x0)
             if first is None:
                 first = v
-            for j, s in enumerate(pending):
-                res[s] = float(info[j][0])
-                cnt[s] += int(info[j][1])
-                V[s] = v[j]                     # device view
+            for j, (s, rj, ij) in enumerate(zip(pending,
+                                                info["residual"].tolist(),
+                                                info["iterations"].tolist())):
+                res[s] = rj
+                cnt[s] += ij
+                V[s] = (v, j)
             retry = [j for j, s in enumerate(pending)
                      if res[s] > accept and r < cap]
             if not retry:
@@ -342,7 +344,7 @@ def _surface_solve(tables, s_all, cfg, y_values, xtal, *, chunk, tol,
         keep = [s for s in sys_idx if s not in split]
         if keep:
             whole = keep == list(sys_idx) and all(
-                V[s].ptr == first[j].ptr for j, s in enumerate(keep))
+                V[s][0] is first and V[s][1] == j for j, s in enumerate(keep))
             Ek = tables(keep, ys) if per and E_sys != keep else E
             if whole and keep == list(range(keep[0], keep[0] + len(keep))):
                 # every system kept from the first attempt, contiguous in
@@ -354,7 +356,8 @@ def _surface_solve(tables, s_all, cfg, y_values, xtal, *, chunk, tol,
                     (len(keep), nyc, cfg.Nx), np.complex128)
                 if not whole:
                     for j, s in enumerate(keep):
-                        vk[j].copy_from(V[s])
+                        src, row = V[s]
+                        vk[j].copy_from(src[row])
                 ew = exit_wave(cfg, xtal, Ek, vk)
                 for j, s in enumerate(keep):
                     out[s, j0:j1] = ew[j]

@@ -825,7 +825,7 @@ __device__ int arnoldi_finish_sys(int s, int restart, int col, cd *H, cd *G,
         active[s] = 0;
         return 0;
     }
-    atomicOr(still, 1);
+    *(volatile int *)still = 1;   // every writer stores the same value
     return 1;
 }
 
@@ -1361,7 +1361,7 @@ __global__ void gmres_cycle_end_kernel(int nsys, const cd *nr, double eps,
     }
     q.col = 0, q.breakdown = 0, q.inner_done = 0;
     active[s] = !q.done;
-    if (!q.done) atomicOr(any, 1);
+    if (!q.done) *(volatile int *)any = 1;
 }
 
 // After the inner loop (which clears a system's flag as its Arnoldi
@@ -1474,7 +1474,6 @@ static Work carve(void *base, int nsys, size_t N, int restart, int Nx) {
     w.inv = (double *)take(sizeof(double) * nsys);
     w.st = (SysState *)take(sizeof(SysState) * nsys);
     w.active = (int *)take(sizeof(int) * nsys);
-    w.still = (int *)take(sizeof(int) * (R + 1));   // per column, `any`
     w.bytes = off;
     return w;
 }
@@ -2249,19 +2248,27 @@ extern "C" int dyf_bragg_solve(const dyf_bragg_args *a, const double *s_dev,
     const dim3 vgrid((unsigned)((N + 255) / 256), nsys);
 
     // The per-system GMRES state lives on the device (set-up, inner-loop
-    // and end-of-cycle kernels); the host reads one word per Arnoldi step
-    // (pipelined, below) and one per restart cycle, and the states once at
-    // the end, through page-locked buffers.
-    static thread_local int *flags_h = nullptr;       // [2] still, [2] any
+    // and end-of-cycle kernels).  The flags the host steers by - per
+    // column "some system still iterates" and per cycle "some system
+    // continues" - are written by those kernels straight into page-locked
+    // host memory (R + 1 words); the host reads a column's word after an
+    // event behind it (pipelined one step late, below), a cycle's after a
+    // stream synchronisation, and the states once at the end.
+    static thread_local int *still_h = nullptr;
+    static thread_local int still_cap = 0;
     static thread_local hipEvent_t still_ev[2] = {nullptr, nullptr};
     static thread_local SysState *st_h = nullptr;
     static thread_local int st_cap = 0;
-    if (!flags_h) {
-        DYF_HIP(hipHostMalloc((void **)&flags_h, 4 * sizeof(int),
-                              hipHostMallocDefault));
+    if (!still_ev[0])
         for (int e = 0; e < 2; ++e)
             DYF_HIP(hipEventCreateWithFlags(&still_ev[e],
                                             hipEventDisableTiming));
+    if (still_cap < R + 1) {
+        if (still_h) DYF_HIP(hipHostFree(still_h));
+        still_h = nullptr, still_cap = 0;
+        DYF_HIP(hipHostMalloc((void **)&still_h, sizeof(int) * (R + 1),
+                              hipHostMallocDefault));
+        still_cap = R + 1;
     }
     if (st_cap < nsys) {
         if (st_h) DYF_HIP(hipHostFree(st_h));
@@ -2270,7 +2277,8 @@ extern "C" int dyf_bragg_solve(const dyf_bragg_args *a, const double *s_dev,
                               hipHostMallocDefault));
         st_cap = nsys;
     }
-    int *still_h = flags_h, *any_h = flags_h + 2;
+    W.still = still_h;
+    volatile const int *flag = still_h;
     auto norm_into = [&](const cd *v, cd *out, const int *active) -> int {
         return dots(v, N, 0, 1, v, N, N, out, 1, W.part, nsys, active,
                     stream);
@@ -2393,12 +2401,10 @@ extern "C" int dyf_bragg_solve(const dyf_bragg_args *a, const double *s_dev,
                               eps, W.still + col);
                 DYF_LAUNCHED("arnoldi_finish_scale_kernel");
             }
-            DYF_HIP(hipMemcpyAsync(&still_h[col & 1], W.still + col, sizeof(int),
-                                   hipMemcpyDeviceToHost, stream));
             DYF_HIP(hipEventRecord(still_ev[col & 1], stream));
             if (col > 0) {
                 DYF_HIP(hipEventSynchronize(still_ev[(col - 1) & 1]));
-                if (!still_h[(col - 1) & 1]) break;
+                if (!flag[col - 1]) break;
             }
         }
         // solve the small least-squares problems, update x and form the
@@ -2423,10 +2429,8 @@ extern "C" int dyf_bragg_solve(const dyf_bragg_args *a, const double *s_dev,
         DYF_LAUNCHED("gmres_cycle_end_kernel");
         // reset the Hessenberg for the next cycle
         DYF_HIP(hipMemsetAsync(W.H, 0, sizeof(cd) * nsys * R * (R + 1), stream));
-        DYF_HIP(hipMemcpyAsync(any_h, W.still + R, sizeof(int),
-                               hipMemcpyDeviceToHost, stream));
         DYF_HIP(hipStreamSynchronize(stream));
-        if (!*any_h) break;
+        if (!flag[R]) break;
     }
     DYF_HIP(hipMemcpyAsync(st_h, W.st, sizeof(SysState) * nsys,
                            hipMemcpyDeviceToHost, stream));
