@@ -39,11 +39,16 @@ def family_110():
 
 
 def random_segments(rng, xtal, grid, n):
-    """n straight lines: direction and Burgers direction drawn from <110>,
-    random sign, |b| = a/sqrt(2), cores uniform in the illuminated
-    Borrmann fan (z in (0, t), |x| < z tan tB, y inside the field of
-    view), snapped half a pixel off the sampled rows; 1 mm long."""
+    """n straight lines: direction and Burgers direction drawn from the
+    crystal's <110> family (taken to the lab frame by xtal.U_lab, as
+    dislocations.burgers_vector does, so g.b is an integer), random sign,
+    |b| = a/sqrt(2), cores uniform in the illuminated Borrmann fan (z in
+    (0, t), |x| < z tan tB, y inside the field of view), snapped half a
+    pixel off the sampled rows; 1 mm long."""
     dirs = family_110()
+    U = np.asarray(xtal.U_lab, float)
+    if not np.array_equal(U, np.eye(3)):
+        dirs = dirs @ U.T
     Ny = grid.geometry()[1]
     segs = []
     for _ in range(n):

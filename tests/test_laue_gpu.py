@@ -51,6 +51,26 @@ def test_many_segments_match_oracle(dev, diamond400, rng):
     _check(got, oracle_exit(segs, s, grid, diamond400), 1e-10, "multi")
 
 
+def test_fractional_winding_table_matches_oracle(dev, diamond400, rng):
+    """The fused e^{iH} table with a segment whose g.b is not an integer
+    (0.74: its winding taken with atan2) beside <110> lines of integer
+    g.b (taken algebraically), FP64 and mixed, against the oracle's
+    exp(iH) march."""
+    grid = LaueGrid(diamond400, 50e-6, 0.25e-6, 8e-6)
+    (ra, rb, b), = geometry_n(diamond400, grid, y0=grid.dx / 2)
+    segs = [(ra, rb, 0.37 * b)] + random_segments(rng, diamond400, grid, 2)
+    gb = [g_dot_b(diamond400.g_vec, s[2]) for s in segs]
+    assert abs(abs(gb[0]) - 0.74) < 1e-9
+    assert all(abs(v - round(v)) < 1e-9 for v in gb[1:])
+    s = _s(diamond400, [-120.0, 0.0, 75.0])
+    want = oracle_exit(segs, s, grid, diamond400)
+    got = to_numpy(solve_dislocation(segs, s, grid, diamond400))
+    _check(got, want, 1e-10, "frac64")
+    got = to_numpy(solve_dislocation(segs, s, grid, diamond400,
+                                     precision=MIXED))
+    _check(got, want, 1e-6, "fracmixed")
+
+
 def test_h_grid_matches_oracle(dev, diamond400, small_grid):
     seg = geometry_n(diamond400, small_grid, y0=small_grid.dx / 2)
     Nx, Ny, dz, _, _ = small_grid.geometry()
