@@ -738,6 +738,20 @@ struct SysState {
 
 __device__ int sys_cols(const SysState *st, int s) { return st[s].col + 1; }
 
+// The words the host steers the loop by: per column "some system still
+// iterates", then per cycle "some system continues" (R + 1 words).  The
+// host reads `host` (page-locked host memory); `gate` is a device copy
+// that lets only the first system to raise a word write it across the
+// bus (with 1312 one-plane systems, one write per column instead of
+// 1312).  Both are cleared at the start of a cycle.
+struct Flags {
+    int *gate, *host;
+};
+
+__device__ __forceinline__ void raise_flag(Flags f, int i) {
+    if (atomicExch(f.gate + i, 1) == 0) *(volatile int *)(f.host + i) = 1;
+}
+
 // LAPACK zlartg convention: [c s; -conj(s) c] [f; g] = [r; 0], c real.
 __device__ void lartg(cd f, cd g, double *c, cd *s, cd *r) {
     const double fa = hypot(f.r, f.i), ga = hypot(g.r, g.i);
@@ -773,7 +787,7 @@ __device__ int arnoldi_finish_sys(int s, int restart, int col, cd *H, cd *G,
                                   cd *S, const cd *h1c, const cd *h2c,
                                   const cd *gp, cd *hc, cd Sc, double nw,
                                   double n0, SysState *st, double *inv,
-                                  int *active, double eps, int *still) {
+                                  int *active, double eps, Flags fl) {
     SysState &q = st[s];
     q.col = col;
     q.h0 = sqrt(n0);
@@ -825,7 +839,7 @@ __device__ int arnoldi_finish_sys(int s, int restart, int col, cd *H, cd *G,
         active[s] = 0;
         return 0;
     }
-    *(volatile int *)still = 1;   // every writer stores the same value
+    raise_flag(fl, col);
     return 1;
 }
 
@@ -879,7 +893,7 @@ __global__ __launch_bounds__(kCgsMaxThreads) void arnoldi_step_kernel(
     cd *__restrict__ V, size_t v_sys, size_t N, int restart, int col,
     const cd *__restrict__ w, cd *__restrict__ h1, cd *__restrict__ h2,
     cd *H, cd *G, cd *S, SysState *st, double *inv, int *active, double eps,
-    int *still) {
+    Flags fl) {
     // LDS (doubles): [NW waves][2 kc] wave sums, h [2 kc], then the first
     // pass's h and the working column [2 (kc + 1)], then the stored
     // rotations [4 col]
@@ -993,7 +1007,7 @@ __global__ __launch_bounds__(kCgsMaxThreads) void arnoldi_step_kernel(
     if (tid == 0)
         red[NW] = arnoldi_finish_sys(s, restart, col, H, G, S, h1l,
                                      (const cd *)(red + hb), gl, h1l, Sc0,
-                                     nn, n0, st, inv, active, eps, still)
+                                     nn, n0, st, inv, active, eps, fl)
                       ? inv[s]
                       : 0.0;
     __syncthreads();
@@ -1007,7 +1021,7 @@ __global__ __launch_bounds__(kCgsMaxThreads) void arnoldi_step_kernel(
 
 using step_fn = void (*)(cd *, size_t, size_t, int, int, const cd *, cd *,
                          cd *, cd *, cd *, cd *, SysState *, double *, int *,
-                         double, int *);
+                         double, Flags);
 
 // Workgroup size of the Arnoldi step for N unknowns (N <= kCgsMaxN):
 // about kCgsPer elements per lane, at most 8
@@ -1264,7 +1278,7 @@ __global__ __launch_bounds__(kSplitThreads) void arnoldi_finish_scale_kernel(
     const cd *__restrict__ w, const cd *__restrict__ pA,
     const cd *__restrict__ pC, const cd *__restrict__ h1,
     const cd *__restrict__ h2, int B, cd *H, cd *G, cd *S, SysState *st,
-    double *inv, int *active, double eps, int *still) {
+    double *inv, int *active, double eps, Flags fl) {
     // LDS (doubles): partials [2 B], then slice 0's h1 / working column
     // [2 (kc + 1)], h2 [2 kc] and rotations [4 col]
     extern __shared__ double red[];
@@ -1294,7 +1308,7 @@ __global__ __launch_bounds__(kSplitThreads) void arnoldi_finish_scale_kernel(
     if (b == 0 && tid == 0)
         arnoldi_finish_sys(s, R, col, H, G, S, h1l, h2l, gl, h1l,
                            S[(size_t)s * (R + 1) + col], nw, n0, st, inv,
-                           active, eps, still);
+                           active, eps, fl);
     const cd *ws = w + (size_t)s * N;
     cd *vn = V + (size_t)s * v_sys + (size_t)kc * N;
 #pragma unroll
@@ -1307,9 +1321,9 @@ __global__ __launch_bounds__(kSplitThreads) void arnoldi_finish_scale_kernel(
 // Start of a restart cycle: inv_s = 1/||M r_s||, S_s = [||M r_s||, 0, ...]
 // (the first column of the Arnoldi basis is M r_s * inv_s).
 __global__ void cycle_start_kernel(int nsys, int R, const cd *nrm,
-                                   double *inv, cd *S, int *still) {
+                                   double *inv, cd *S, Flags fl) {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (i <= (size_t)R) still[i] = 0;   // R per-column words, then `any`
+    if (i <= (size_t)R) fl.gate[i] = 0, fl.host[i] = 0;   // R columns, `any`
     if (i >= (size_t)nsys * (R + 1)) return;
     const int s = (int)(i / (R + 1)), q = (int)(i % (R + 1));
     const double t = sqrt(nrm[s].r);
@@ -1343,7 +1357,8 @@ __global__ void gmres_setup_kernel(int nsys, const cd *nb, const cd *nmb,
 // and the update of the inner tolerance; then the next cycle's flags.
 // `any` (cleared at the cycle's start) is set while a system continues.
 __global__ void gmres_cycle_end_kernel(int nsys, const cd *nr, double eps,
-                                       SysState *st, int *active, int *any) {
+                                       SysState *st, int *active, Flags fl,
+                                       int any) {
     const int s = blockIdx.x * 64 + threadIdx.x;
     if (s >= nsys) return;
     SysState &q = st[s];
@@ -1361,7 +1376,7 @@ __global__ void gmres_cycle_end_kernel(int nsys, const cd *nr, double eps,
     }
     q.col = 0, q.breakdown = 0, q.inner_done = 0;
     active[s] = !q.done;
-    if (!q.done) *(volatile int *)any = 1;
+    if (!q.done) raise_flag(fl, any);
 }
 
 // After the inner loop (which clears a system's flag as its Arnoldi
@@ -1426,7 +1441,9 @@ struct Work {
         *part, *pA, *pB, *pC;
     double *inv;
     SysState *st;
-    int *active, *still;
+    int *active, *gate;
+    Flags flags() const { return Flags{gate, still}; }
+    int *still;   // page-locked host words, set by dyf_bragg_solve
     size_t bytes;
 };
 
@@ -1474,6 +1491,7 @@ static Work carve(void *base, int nsys, size_t N, int restart, int Nx) {
     w.inv = (double *)take(sizeof(double) * nsys);
     w.st = (SysState *)take(sizeof(SysState) * nsys);
     w.active = (int *)take(sizeof(int) * nsys);
+    w.gate = (int *)take(sizeof(int) * (R + 1));   // per column, `any`
     w.bytes = off;
     return w;
 }
@@ -2346,7 +2364,7 @@ extern "C" int dyf_bragg_solve(const dyf_bragg_args *a, const double *s_dev,
         if ((rc = apply_precon(k, W.pinv, W.r, W.w, stream))) return rc;
         if ((rc = norm_into(W.w, W.nrm, W.active))) return rc;
         cycle_start_kernel<<<sgrid, 256, 0, stream>>>(nsys, R, W.nrm, W.inv,
-                                                      W.S, W.still);
+                                                      W.S, W.flags());
         DYF_LAUNCHED("cycle_start_kernel");
         scale_copy_kernel<<<vgrid, 256, 0, stream>>>(W.w, N, W.V, Vsys, W.inv,
                                                      N, W.active);
@@ -2368,7 +2386,7 @@ extern "C" int dyf_bragg_solve(const dyf_bragg_args *a, const double *s_dev,
                     sizeof(double) * (2 * (T / 64) + 8) * (size_t)(col + 1);
                 step_kernel(N)<<<nsys, T, lds, stream>>>(
                     W.V, Vsys, N, R, col, W.w, W.h1, W.h2, W.H, W.G, W.S,
-                    W.st, W.inv, W.active, eps, W.still + col);
+                    W.st, W.inv, W.active, eps, W.flags());
                 DYF_LAUNCHED("arnoldi_step_kernel");
             } else {
                 // CGS2 over slices of kSplitLen elements (three passes
@@ -2398,7 +2416,7 @@ extern "C" int dyf_bragg_solve(const dyf_bragg_args *a, const double *s_dev,
                     sizeof(double) * (2 * B + 4 * kc + 2 + 4 * col),
                     stream>>>(W.V, Vsys, N, R, col, W.w, W.pA, W.pC, W.h1,
                               W.h2, B, W.H, W.G, W.S, W.st, W.inv, W.active,
-                              eps, W.still + col);
+                              eps, W.flags());
                 DYF_LAUNCHED("arnoldi_finish_scale_kernel");
             }
             DYF_HIP(hipEventRecord(still_ev[col & 1], stream));
@@ -2425,7 +2443,7 @@ extern "C" int dyf_bragg_solve(const dyf_bragg_args *a, const double *s_dev,
         DYF_LAUNCHED("sub_kernel");
         if ((rc = norm_into(W.r, W.nrm, W.active))) return rc;
         gmres_cycle_end_kernel<<<tgrid, 64, 0, stream>>>(
-            nsys, W.nrm, eps, W.st, W.active, W.still + R);
+            nsys, W.nrm, eps, W.st, W.active, W.flags(), R);
         DYF_LAUNCHED("gmres_cycle_end_kernel");
         // reset the Hessenberg for the next cycle
         DYF_HIP(hipMemsetAsync(W.H, 0, sizeof(cd) * nsys * R * (R + 1), stream));
