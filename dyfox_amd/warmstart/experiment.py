@@ -158,6 +158,9 @@ def main(argv=None):
     ap.add_argument("--eval-only", action="store_true",
                     help="evaluate the network in --weights on --test "
                          "held-out instances (no training)")
+    ap.add_argument("--joint-max-um", type=float, default=50.0,
+                    help="evaluate the reference's joint-chunk policy on "
+                         "held-out instances up to this thickness")
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
     t0 = time.perf_counter()
@@ -167,11 +170,12 @@ def main(argv=None):
 
     if a.eval_only:
         rep = evaluate_saved(a.weights, a.test,
-                             t_range=(a.tmin * 1e-6, a.tmax * 1e-6), log=log)
+                             t_range=(a.tmin * 1e-6, a.tmax * 1e-6), log=log,
+                             joint_max_t=a.joint_max_um * 1e-6)
     else:
         rep = run(a.train, a.test, epochs=a.epochs,
                   t_range=(a.tmin * 1e-6, a.tmax * 1e-6), log=log,
-                  weights=a.weights)
+                  weights=a.weights, joint_max_t=a.joint_max_um * 1e-6)
     line = json.dumps(rep)
     if a.out:
         with open(a.out, "w") as f:
