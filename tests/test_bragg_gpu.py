@@ -304,6 +304,37 @@ def test_retry_and_split_policy_is_followed(dev, diamond400, g):
     assert _rel(out, want) < 1e-8
 
 
+def test_sliced_arnoldi_step_follows_the_policy(dev, diamond400, g):
+    """An 8-plane joint chunk (8 x 324 = 2592 unknowns, more than one
+    workgroup holds: the Arnoldi step over three 1024-element slices),
+    retried to the cap and then split into 4-plane halves (which take the
+    one-workgroup step): the same retries, splits and surface as the
+    oracle."""
+    cfg = BraggConfig(diamond400, t_crystal=6e-6, dx=0.2e-6, half_x=2e-6)
+    assert 8 * cfg.Nx > 2048
+    sd = float(deviation_from_urad(-6.5, diamond400))
+    seg = threading_segment("Edge", +1, diamond400.b_mag, float(g[2]),
+                            y0=-0.3e-6)
+    ys = (np.arange(8) - 4) * cfg.dx
+    kw = dict(accept=1e-30, restart=4, restart_cap=16, per_plane=1,
+              max_depth=1, tries=3)
+    out, info = solve_surface_batch(seg, sd, cfg, ys, diamond400, g,
+                                    planes="joint", **kw)
+    want, winfo = OB.solve_surface_batch(seg, sd, cfg, ys, diamond400, g,
+                                         **kw)
+    print(info, winfo)
+    assert info["retries"] > 0 and info["splits"] == 1
+    for k in ("retries", "splits", "unconverged_chunks"):
+        assert info[k] == winfo[k], k
+    assert _rel(out, want) < 1e-8
+    # the same chunk to tol 1e-10 under the default policy: SciPy's
+    # iteration count and the oracle's surface
+    out, info = solve_surface_batch(seg, sd, cfg, ys, diamond400, g)
+    want, winfo = OB.solve_surface_batch(seg, sd, cfg, ys, diamond400, g)
+    assert info["converged"] and info["iterations"] == winfo["iterations"]
+    assert _rel(out, want) < 1e-8
+
+
 def test_warm_start_reaches_the_same_answer_in_fewer_iterations(dev,
                                                                 diamond400, g):
     cfg = BraggConfig(diamond400, t_crystal=6e-6, dx=0.2e-6, half_x=2e-6)
