@@ -109,6 +109,7 @@ extern "C" int dyf_laue_batch_prepare(const dyf_laue_instance *in, int n,
     if (!work || !meta || work_bytes < w || meta_bytes < m)
         return fail(DYF_EINVAL, "workspace too small (%zu/%zu, need %zu/%zu)",
                     work_bytes, meta_bytes, w, m);
+    if ((rc = ensure_log_table())) return rc;   // before any capture of run
     for (int i = 0; i < n; ++i)
         if (in[i].n_e > 0 && (!in[i].s_dev || !in[i].out))
             return fail(DYF_EINVAL, "instance %d: NULL angles or output", i);

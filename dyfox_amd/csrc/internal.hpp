@@ -7,8 +7,14 @@
 
 namespace dyf {
 
+// Builds the device-wide table of log_tab once per device (a kernel and a
+// synchronisation on first use); call before enqueueing any kernel that
+// evaluates segments, outside stream capture.
+int ensure_log_table();
+
 // e^{iH} table of one Laue instance from segments already on the device
-// (the batched path; graph-capturable: kernels only).
+// (the batched path; graph-capturable: kernels only; ensure_log_table
+// must have run on the device).
 int launch_phase_segments_dev(const dyf_laue_args *a, const PhaseLayout &L,
                               int prec, const SegParams *segs_dev, int nseg,
                               double a2, void *phase, hipStream_t stream);

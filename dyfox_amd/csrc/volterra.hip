@@ -7,6 +7,8 @@
 // Segments live in constant memory (uniform across every wave, so every
 // read is a scalar load); calls on one device are serialised by stream
 // order, which is what makes one symbol per library enough.
+#include <mutex>
+
 #include "internal.hpp"
 
 namespace dyf {
@@ -16,6 +18,33 @@ constexpr int kMaxRows = 2048;
 __constant__ SegParams c_segs[kMaxSeg];
 __constant__ double c_yvals[kMaxRows];
 
+// The log_tab table, built once per device by fill_log_table and copied
+// into LDS by every workgroup that evaluates segments (a copy of 2 KB
+// instead of 128 library logs per workgroup).
+__device__ LogEntry g_logtab[kLogTab];
+
+__global__ void log_table_init_kernel() { fill_log_table(g_logtab); }
+
+__device__ __forceinline__ void load_log_table(LogEntry *tab) {
+    for (int j = threadIdx.x; j < kLogTab; j += blockDim.x) tab[j] = g_logtab[j];
+}
+
+int ensure_log_table() {
+    static std::mutex mu;
+    static bool done[64] = {};
+    int dev = 0;
+    DYF_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(DYF_EINVAL, "device %d", dev);
+    std::lock_guard<std::mutex> lock(mu);
+    if (done[dev]) return DYF_OK;
+    // synchronous, so every stream sees the table from the first use on
+    log_table_init_kernel<<<1, kLogTab, 0, nullptr>>>();
+    DYF_LAUNCHED("log_table_init_kernel");
+    DYF_HIP(hipStreamSynchronize(nullptr));
+    done[dev] = true;
+    return DYF_OK;
+}
+
 static int upload_segments(const double *segs, int nseg, const double *g,
                            double nu, int *kept, hipStream_t stream) {
     if (nseg < 0 || (nseg > 0 && !segs))
@@ -24,6 +53,7 @@ static int upload_segments(const double *segs, int nseg, const double *g,
         return fail(DYF_EINVAL, "%d segments exceed the limit of %d", nseg,
                     kMaxSeg);
     if (!g) return fail(DYF_EINVAL, "g_vec is NULL");
+    if (int rc = ensure_log_table()) return rc;
     SegParams host[kMaxSeg];
     *kept = prepare_segments(segs, nseg, g, nu, host);
     // Synchronous: the host array is gone once this call returns, and a
@@ -61,21 +91,33 @@ __global__ __launch_bounds__(256) void volterra_grid_kernel(
 // Thread t of a (row, step block) covers x = lane*C + i with i = t / 64,
 // lane = t % 64, for kPhaseZ consecutive steps, and writes element t of
 // each of those (row, step) blocks.  Columns past Nx (the lane padding)
-// are written as zeros.
+// are written as zeros.  The grid is flat over (row, step block, t): a
+// tile's C * 64 threads are whole wavefronts, so no wavefront is idle
+// (with 256-thread blocks per tile, 1088 = 17 x 64 threads left three of
+// every twenty wavefronts without work).
 constexpr int kPhaseZ = 4;
+
+static dim3 phase_table_grid(const PhaseLayout &L) {
+    const size_t nzb = (L.nsteps + kPhaseZ - 1) / kPhaseZ;
+    const size_t n = (size_t)L.nrows * nzb * L.C * kWave;
+    return dim3((unsigned)((n + 255) / 256));
+}
 
 template <typename T>
 __device__ __forceinline__ void phase_table_points(
     cx<T> *__restrict__ E, const SegParams *__restrict__ segs, int nseg,
-    int C, int Nx, int Ny, int nsteps, int iy_lo, double dx, double dz,
-    double a2) {
+    int C, int Nx, int Ny, int nsteps, int nrows, int iy_lo, double dx,
+    double dz, double a2) {
     __shared__ LogEntry tab[kLogTab];
-    fill_log_table(tab);
+    load_log_table(tab);
     __syncthreads();
-    const int t = blockIdx.x * 256 + threadIdx.x;
-    const int k0 = blockIdx.y * kPhaseZ;
-    const int r = blockIdx.z;
-    if (t >= C * kWave) return;
+    const int per = C * kWave, nzb = (nsteps + kPhaseZ - 1) / kPhaseZ;
+    const size_t g = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t tile = g / per;
+    if (tile >= (size_t)nrows * nzb) return;
+    const int t = (int)(g - tile * per);
+    const int r = (int)(tile / nzb);
+    const int k0 = (int)(tile - (size_t)r * nzb) * kPhaseZ;
     const int i = t / kWave, lane = t % kWave;
     const int x = lane * C + i;
     const size_t stride = (size_t)C * kWave;
@@ -108,9 +150,9 @@ __device__ __forceinline__ void phase_table_points(
 template <typename T>
 __global__ __launch_bounds__(256) void phase_from_segments_kernel(
     cx<T> *__restrict__ E, int nseg, int C, int Nx, int Ny, int nsteps,
-    int iy_lo, double dx, double dz, double a2) {
-    phase_table_points<T>(E, c_segs, nseg, C, Nx, Ny, nsteps, iy_lo, dx, dz,
-                          a2);
+    int nrows, int iy_lo, double dx, double dz, double a2) {
+    phase_table_points<T>(E, c_segs, nseg, C, Nx, Ny, nsteps, nrows, iy_lo,
+                          dx, dz, a2);
 }
 
 // Same table with the segments read from device memory (uniform loads),
@@ -119,26 +161,25 @@ __global__ __launch_bounds__(256) void phase_from_segments_kernel(
 template <typename T>
 __global__ __launch_bounds__(256) void phase_segments_dev_kernel(
     cx<T> *__restrict__ E, const SegParams *__restrict__ segs, int nseg,
-    int C, int Nx, int Ny, int nsteps, int iy_lo, double dx, double dz,
-    double a2) {
-    phase_table_points<T>(E, segs, nseg, C, Nx, Ny, nsteps, iy_lo, dx, dz,
-                          a2);
+    int C, int Nx, int Ny, int nsteps, int nrows, int iy_lo, double dx,
+    double dz, double a2) {
+    phase_table_points<T>(E, segs, nseg, C, Nx, Ny, nsteps, nrows, iy_lo, dx,
+                          dz, a2);
 }
 
 int launch_phase_segments_dev(const dyf_laue_args *a, const PhaseLayout &L,
                               int prec, const SegParams *segs, int nseg,
                               double a2, void *phase, hipStream_t stream) {
     if (L.nrows == 0) return DYF_OK;
-    dim3 grid((L.C * kWave + 255) / 256,
-              (L.nsteps + kPhaseZ - 1) / kPhaseZ, L.nrows);
+    const dim3 grid = phase_table_grid(L);
     if (prec == DYF_FP64)
         phase_segments_dev_kernel<double><<<grid, 256, 0, stream>>>(
             (cx<double> *)phase, segs, nseg, L.C, a->Nx, a->Ny, L.nsteps,
-            L.iy_lo, a->dx, a->dz, a2);
+            L.nrows, L.iy_lo, a->dx, a->dz, a2);
     else
         phase_segments_dev_kernel<float><<<grid, 256, 0, stream>>>(
             (cx<float> *)phase, segs, nseg, L.C, a->Nx, a->Ny, L.nsteps,
-            L.iy_lo, a->dx, a->dz, a2);
+            L.nrows, L.iy_lo, a->dx, a->dz, a2);
     DYF_LAUNCHED("phase_segments_dev_kernel");
     return DYF_OK;
 }
@@ -202,7 +243,7 @@ __global__ __launch_bounds__(256) void bragg_phase_kernel(
     cx<double> *__restrict__ E, int nseg, int Nx, int ny, double dx,
     double dz, double a2) {
     __shared__ LogEntry tab[kLogTab];
-    fill_log_table(tab);
+    load_log_table(tab);
     __syncthreads();
     const int ix = blockIdx.x * 256 + threadIdx.x;
     const int k = blockIdx.y, j = blockIdx.z;
@@ -293,17 +334,16 @@ extern "C" int dyf_laue_phase_from_segments(const dyf_laue_args *a, int prec,
     int kept = 0;
     rc = upload_segments(segs, nseg, g, nu, &kept, stream);
     if (rc) return rc;
-    dim3 grid((L.C * kWave + 255) / 256,
-              (L.nsteps + kPhaseZ - 1) / kPhaseZ, L.nrows);
+    const dim3 grid = phase_table_grid(L);
     const double a2 = a_core * a_core;
     if (prec == DYF_FP64)
         phase_from_segments_kernel<double><<<grid, 256, 0, stream>>>(
-            (cx<double> *)phase, kept, L.C, a->Nx, a->Ny, L.nsteps, L.iy_lo,
-            a->dx, a->dz, a2);
+            (cx<double> *)phase, kept, L.C, a->Nx, a->Ny, L.nsteps, L.nrows,
+            L.iy_lo, a->dx, a->dz, a2);
     else
         phase_from_segments_kernel<float><<<grid, 256, 0, stream>>>(
-            (cx<float> *)phase, kept, L.C, a->Nx, a->Ny, L.nsteps, L.iy_lo,
-            a->dx, a->dz, a2);
+            (cx<float> *)phase, kept, L.C, a->Nx, a->Ny, L.nsteps, L.nrows,
+            L.iy_lo, a->dx, a->dz, a2);
     DYF_LAUNCHED("phase_from_segments_kernel");
     return DYF_OK;
 }
