@@ -138,3 +138,38 @@ def test_wavecache_key_and_path_match_reference(i):
     assert repr(wc) == rec["repr"]
     # the key changes with any input (a planted change of one deviation)
     assert wc.key(segs, s * (1 + 1e-12) + 1e-3, grid, xt, prec) != rec["key"]
+
+
+def test_wavecache_keys_separate_the_physics(diamond400):
+    """Every input that changes the exit waves changes the key (the
+    reference's key-separation test, tests/test_laue_gpu.py:275-288):
+    segments, deviations, grid, crystal, precision and stored dtype; the
+    same inputs give the same key."""
+    from dyfox_amd.dislocations import burgers_vector, straight_segment
+    grid = LaueGrid(diamond400, 40e-6, 0.25e-6, 8e-6)
+    b = burgers_vector(np.eye(3), [1, 0, -1], diamond400.b_mag)
+    segs = straight_segment(np.array([1.0, 0.0, 1.0]) / np.sqrt(2), b,
+                            [0.0, grid.dx / 2, 20e-6], 400e-6)
+    s = np.array([0.0, 1.2e6])
+    wc = WaveCache("/tmp/waves")
+    k = wc.key(segs, s, grid, diamond400)
+    assert k == WaveCache("/elsewhere").key(segs, s, grid, diamond400)
+    flipped = straight_segment(np.array([1.0, 0.0, 1.0]) / np.sqrt(2), -b,
+                               [0.0, grid.dx / 2, 20e-6], 400e-6)
+    d220 = CrystalParams.from_reflection(DIAMOND, (2, 2, 0), 12.0,
+                                         f_prime=0.003, f_dprime=0.002)
+    others = [
+        wc.key(flipped, s, grid, diamond400),
+        wc.key([], s, grid, diamond400),
+        wc.key(segs, s[:1], grid, diamond400),
+        wc.key(segs, s + 1.0, grid, diamond400),
+        wc.key(segs, s, grid.replace(beam_thickness=2e-6), diamond400),
+        wc.key(segs, s, LaueGrid(diamond400, 40e-6, 0.25e-6, 10e-6),
+               diamond400),
+        wc.key(segs, s, grid, d220),
+        wc.key(segs, s, grid, diamond400, precision("fp32")),
+        WaveCache("/tmp/waves", store_dtype=np.complex128).key(
+            segs, s, grid, diamond400),
+        WaveCache("/tmp/waves", version=8).key(segs, s, grid, diamond400),
+    ]
+    assert len(set(others + [k])) == len(others) + 1
