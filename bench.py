@@ -70,6 +70,10 @@ def parse():
                          "data and weight gradients of every 3x3 layer at "
                          "batch 32) with HIP events, print one line each, "
                          "and exit")
+    ap.add_argument("--probe-ensemble", action="store_true",
+                    help="run only the 256-instance ensembles (FP64, then "
+                         "mixed: autotune, capture, one replay each), for "
+                         "a kernel trace, and exit")
     ap.add_argument("--probe", action="store_true",
                     help="for counter runs: one phase build and one march "
                          "per precision, then a 1 GiB copy kernel (the "
@@ -620,6 +624,14 @@ def main():
         return probe_laue()
     if args.probe_gemm:
         return probe_gemm()
+    if args.probe_ensemble:
+        from dyfox_amd.backend import device
+        device()
+        r = time_ensembles()
+        for name, v in r.items():
+            print(f"probe-ensemble {name}: {v['seconds']:.3f} s, "
+                  f"{v['value']:.3e} pixel-steps/s", flush=True)
+        return None
     if args.probe_matvec:
         from dyfox_amd import _native as N
         from dyfox_amd.backend import device
