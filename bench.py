@@ -716,7 +716,7 @@ def main():
 
     achieved = FLOPS_PER_PIXEL_STEP * work / (m64 * 1e-3) / 1e12
     issued = ISSUED_FLOPS_PER_PIXEL_STEP * work / (m64 * 1e-3) / 1e12
-    roof = {"kernel": "laue_fast_kernel<17, GRID, FP64>",
+    roof = {"kernel": "laue_fast_inplace_kernel<17, FP64>",
             "bound": "fp64_valu", "achieved": achieved,
             "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_TFLOPS,

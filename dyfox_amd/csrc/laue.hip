@@ -368,6 +368,24 @@ using gptr = const T __attribute__((address_space(1))) *;
 template <typename T>
 using gptr_w = T __attribute__((address_space(1))) *;
 
+// Buffer-resource word 3 for raw (unswizzled, stride 0) buffers on gfx9
+// parts: 32-bit data format, no index stride.
+constexpr int kBufferWord3 = 0x00020000;
+
+template <typename T>
+__device__ __forceinline__ cx<T> load_cx(__amdgpu_buffer_rsrc_t rs, int voff,
+                                         int soff) {
+    if constexpr (sizeof(T) == 8) {
+        using u4 = unsigned __attribute__((ext_vector_type(4)));
+        const u4 v = __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, 0);
+        return __builtin_bit_cast(cx<T>, v);
+    } else {
+        using u2 = unsigned __attribute__((ext_vector_type(2)));
+        const u2 v = __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 0);
+        return __builtin_bit_cast(cx<T>, v);
+    }
+}
+
 template <int C, int PREC, bool ROLL = false>
 __device__ __forceinline__ void fast_row_unrolled(const FastArgs &a, int r,
                                                   int ie) {
@@ -388,7 +406,14 @@ __device__ __forceinline__ void fast_row_unrolled(const FastArgs &a, int r,
         Xi[i] = Yr[i] = Yi[i] = T(0);
         if constexpr (COMP) Xlr[i] = Xli[i] = Ylr[i] = Yli[i] = T(0);
     }
-    gptr<cx<T>> E = (gptr<cx<T>>)a.phase + (size_t)r * a.nsteps * kSlice + lane;
+    // the row's table through a buffer resource, the lane's byte offset
+    // the only vector operand (as in fast_row_inplace)
+    constexpr int kEl = (int)sizeof(cx<T>);
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)((const cx<T> *)a.phase + (size_t)r * a.nsteps * kSlice), 0,
+        a.nsteps * kSlice * kEl, kBufferWord3);
+    const int loff = lane * kEl;
+    auto slice = [&](int k, int i) { return load_cx<T>(rs, loff, (k * C + i) * kWave * kEl); };
 
     // ROLL: the e^{iH} slice is prefetched on a rolling basis, as in
     // fast_row (column i's value for step k+1 is loaded as soon as column
@@ -397,20 +422,16 @@ __device__ __forceinline__ void fast_row_unrolled(const FastArgs &a, int r,
     if constexpr (ROLL) {
 #pragma unroll
         for (int i = 0; i < C; ++i) {
-            Ec[i] = cx<T>{E[i * kWave].r, E[i * kWave].i};
+            Ec[i] = slice(0, i);
             if constexpr (sizeof(T) == 4) __builtin_amdgcn_sched_barrier(0);
         }
     }
 
     auto step = [&](int k) {
-        gptr<cx<T>> Ek = E + (size_t)k * kSlice;
-        if constexpr (ROLL) {
-            const int kn = k + 1 < a.nsteps ? k + 1 : k;
-            Ek = E + (size_t)kn * kSlice;
-        } else {
+        const int kn = ROLL ? (k + 1 < a.nsteps ? k + 1 : k) : k;
+        if constexpr (!ROLL) {
 #pragma unroll
-            for (int i = 0; i < C; ++i)
-                Ec[i] = cx<T>{Ek[i * kWave].r, Ek[i * kWave].i};
+            for (int i = 0; i < C; ++i) Ec[i] = slice(k, i);
         }
         T cr, ci, br, bi;
         const size_t ci_ = (size_t)__builtin_amdgcn_readfirstlane(ie) * a.nsteps + k;
@@ -445,7 +466,7 @@ __device__ __forceinline__ void fast_row_unrolled(const FastArgs &a, int r,
             const T ui = fmaT(er, xi, -ei * xr);
             if constexpr (ROLL) {
                 asm volatile("" ::: "memory");
-                Ec[i] = cx<T>{Ek[i * kWave].r, Ek[i * kWave].i};
+                Ec[i] = slice(kn, i);
                 if constexpr (sizeof(T) == 4) __builtin_amdgcn_sched_barrier(0);
             }
             if constexpr (!COMP) {
@@ -509,6 +530,221 @@ __device__ __forceinline__ void fast_row_unrolled(const FastArgs &a, int r,
             o->r = (float)o_r, o->i = (float)o_i;
         }
     }
+}
+
+// fast_row for the grid mode with the state updated in place, in two
+// passes per step, so the rolled loop needs no register copies.
+//
+// Column i of a step reads X[i+1] and Y[i-1] only.  A pass in ascending
+// column order can overwrite X[i] (its last reader, column i-1, has run)
+// but not Y[i] (column i+1 still reads it); descending order is the other
+// way round.  So one pass updates one field in place and leaves, in the
+// register that held column i's e^{iH}, the product the other field needs
+// (u = e^{-iH} X[i+1] or t = e^{iH} Y[i-1]); the second pass runs in the
+// opposite order, updates the other field in place from it, and loads the
+// next step's e^{iH} of column i into that register.  Consecutive steps
+// swap the pass order (X ascending then Y descending; Y descending then X
+// ascending), so every column's next e^{iH} is loaded a whole step before
+// its use.  Each value is formed by the same operations as in fast_row,
+// so the results are bit-identical.
+template <int C, int PREC, int U>
+__device__ __forceinline__ void fast_row_inplace(const FastArgs &a, int r,
+                                                 int ie) {
+    using S = StateT<PREC>;
+    using T = typename S::T;
+    constexpr bool COMP = S::comp;
+    constexpr int kSlice = C * kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int iy = a.iy_lo + r;
+
+    T Xr[C], Xi[C], Yr[C], Yi[C];
+    T Xlr[COMP ? C : 1], Xli[COMP ? C : 1], Ylr[COMP ? C : 1], Yli[COMP ? C : 1];
+#pragma unroll
+    for (int i = 0; i < C; ++i) {
+        const int x = lane * C + i;
+        Xr[i] = x < a.Nx ? (T)beam_profile(x, a.Nx, a.dx, a.sheet_hw) : T(0);
+        Xi[i] = Yr[i] = Yi[i] = T(0);
+        if constexpr (COMP) Xlr[i] = Xli[i] = Ylr[i] = Yli[i] = T(0);
+    }
+    // The row's table is read through a buffer resource (scalar registers)
+    // with the lane's byte offset as the only vector operand: the column
+    // and step offsets are scalar, so no 64-bit addresses are held or
+    // advanced in vector registers.  The row spans nsteps * C * 64 entries
+    // (8.5 MB at Nx 1056, Nz 490), well inside the 32-bit offset range.
+    constexpr int kEl = (int)sizeof(cx<T>);
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)((const cx<T> *)a.phase + (size_t)r * a.nsteps * kSlice), 0,
+        a.nsteps * kSlice * kEl, kBufferWord3);
+    const int loff = lane * kEl;
+    auto slice = [&](int k, int i) { return load_cx<T>(rs, loff, (k * C + i) * kWave * kEl); };
+    // column i's e^{iH}, or between the two passes the product the second
+    // pass needs
+    T Sr[C], Si[C];
+#pragma unroll
+    for (int i = 0; i < C; ++i) {
+        const cx<T> e = slice(0, i);
+        Sr[i] = e.r, Si[i] = e.i;
+    }
+
+    auto coefs = [&](int k, T &cr, T &ci, T &br, T &bi) {
+        const size_t at = (size_t)__builtin_amdgcn_readfirstlane(ie) * a.nsteps + k;
+        if constexpr (sizeof(T) == 8) {
+            kptr<CoefD> c = (kptr<CoefD>)a.cd + at;
+            cr = c->cr, ci = c->ci, br = c->br, bi = c->bi;
+        } else {
+            kptr<CoefF> c = (kptr<CoefF>)a.cf + at;
+            cr = c->cr, ci = c->ci, br = c->br, bi = c->bi;
+        }
+    };
+    // X'[i] = X[i+1] + c t (and its compensation), in place
+    auto new_x = [&](int i, T xr, T xi, T xlr, T xli, T tr, T ti, T cr, T ci) {
+        if constexpr (!COMP) {
+            Xr[i] = fmaT(cr, tr, fmaT(-ci, ti, xr));
+            Xi[i] = fmaT(cr, ti, fmaT(ci, tr, xi));
+        } else {
+            const T dr = fmaT(cr, tr, fmaT(-ci, ti, xlr));
+            const T di = fmaT(cr, ti, fmaT(ci, tr, xli));
+            const T sr = xr + dr, si = xi + di;
+            Xlr[i] = dr - (sr - xr), Xli[i] = di - (si - xi);
+            Xr[i] = sr, Xi[i] = si;
+        }
+    };
+    // Y'[i] = Y[i-1] + b u (and its compensation), in place
+    auto new_y = [&](int i, T yr, T yi, T ylr, T yli, T ur, T ui, T br, T bi) {
+        if constexpr (!COMP) {
+            Yr[i] = fmaT(br, ur, fmaT(-bi, ui, yr));
+            Yi[i] = fmaT(br, ui, fmaT(bi, ur, yi));
+        } else {
+            const T dr = fmaT(br, ur, fmaT(-bi, ui, ylr));
+            const T di = fmaT(br, ui, fmaT(bi, ur, yli));
+            const T sr = yr + dr, si = yi + di;
+            Ylr[i] = dr - (sr - yr), Yli[i] = di - (si - yi);
+            Yr[i] = sr, Yi[i] = si;
+        }
+    };
+
+    // one step; XFIRST: X pass ascending, then Y pass descending
+    auto step = [&](int k, auto xfirst) {
+        constexpr bool XFIRST = decltype(xfirst)::value;
+        T cr, ci, br, bi;
+        coefs(k, cr, ci, br, bi);
+        const int kn = k + 1 < a.nsteps ? k + 1 : k;
+        // inflow across lane boundaries (old values)
+        const T xin_r = from_right(Xr[0], lane), xin_i = from_right(Xi[0], lane);
+        const T yin_r = from_left(Yr[C - 1], lane), yin_i = from_left(Yi[C - 1], lane);
+        T xinl_r = T(0), xinl_i = T(0), yinl_r = T(0), yinl_i = T(0);
+        if constexpr (COMP) {
+            xinl_r = from_right(Xlr[0], lane), xinl_i = from_right(Xli[0], lane);
+            yinl_r = from_left(Ylr[C - 1], lane), yinl_i = from_left(Yli[C - 1], lane);
+        }
+        // sources of column i: X from x+1, Y from x-1 (old values while
+        // the pass order keeps them unwritten)
+        auto xs = [&](int i, T &r_, T &i_, T &lr, T &li) {
+            r_ = i + 1 < C ? Xr[i + 1] : xin_r, i_ = i + 1 < C ? Xi[i + 1] : xin_i;
+            if constexpr (COMP)
+                lr = i + 1 < C ? Xlr[i + 1] : xinl_r, li = i + 1 < C ? Xli[i + 1] : xinl_i;
+        };
+        auto ys = [&](int i, T &r_, T &i_, T &lr, T &li) {
+            r_ = i > 0 ? Yr[i - 1] : yin_r, i_ = i > 0 ? Yi[i - 1] : yin_i;
+            if constexpr (COMP)
+                lr = i > 0 ? Ylr[i - 1] : yinl_r, li = i > 0 ? Yli[i - 1] : yinl_i;
+        };
+        auto first = [&](int i) {
+            T xr, xi, xlr = T(0), xli = T(0), yr, yi, ylr = T(0), yli = T(0);
+            xs(i, xr, xi, xlr, xli);
+            ys(i, yr, yi, ylr, yli);
+            const T er = Sr[i], ei = Si[i];
+            const T tr = fmaT(er, yr, -ei * yi);    // t = e^{iH} y
+            const T ti = fmaT(er, yi, ei * yr);
+            const T ur = fmaT(er, xr, ei * xi);     // u = e^{-iH} x
+            const T ui = fmaT(er, xi, -ei * xr);
+            if constexpr (XFIRST) {
+                new_x(i, xr, xi, xlr, xli, tr, ti, cr, ci);
+                Sr[i] = ur, Si[i] = ui;
+            } else {
+                new_y(i, yr, yi, ylr, yli, ur, ui, br, bi);
+                Sr[i] = tr, Si[i] = ti;
+            }
+        };
+        auto second = [&](int i) {
+            if constexpr (XFIRST) {
+                T yr, yi, ylr = T(0), yli = T(0);
+                ys(i, yr, yi, ylr, yli);
+                new_y(i, yr, yi, ylr, yli, Sr[i], Si[i], br, bi);
+            } else {
+                T xr, xi, xlr = T(0), xli = T(0);
+                xs(i, xr, xi, xlr, xli);
+                new_x(i, xr, xi, xlr, xli, Sr[i], Si[i], cr, ci);
+            }
+            asm volatile("" ::: "memory");
+            const cx<T> e = slice(kn, i);
+            Sr[i] = e.r, Si[i] = e.i;
+        };
+        if constexpr (XFIRST) {
+#pragma unroll
+            for (int i = 0; i < C; ++i) first(i);
+#pragma unroll
+            for (int i = C - 1; i >= 0; --i) second(i);
+        } else {
+#pragma unroll
+            for (int i = C - 1; i >= 0; --i) first(i);
+#pragma unroll
+            for (int i = 0; i < C; ++i) second(i);
+        }
+    };
+    // U steps per iteration (U / 2 of each pass order)
+    using XF = std::integral_constant<bool, true>;
+    using YF = std::integral_constant<bool, false>;
+    int k = 0;
+    if constexpr (U > 2) {
+        for (; k + U <= a.nsteps; k += U) {
+#pragma unroll
+            for (int kk = 0; kk < U; kk += 2) {
+                step(k + kk, XF{});
+                step(k + kk + 1, YF{});
+            }
+        }
+    }
+    for (; k + 2 <= a.nsteps; k += 2) {
+        step(k, XF{});
+        step(k + 1, YF{});
+    }
+    if (k < a.nsteps) step(k, XF{});
+
+    gptr<double> qN = (gptr<double>)a.qN;
+    const double qr = qN[2 * ie], qi = qN[2 * ie + 1];
+    const size_t row = ((size_t)ie * a.Ny + iy) * a.Nx;
+#pragma unroll
+    for (int i = 0; i < C; ++i) {
+        const int x = lane * C + i;
+        if (x >= a.Nx) continue;
+        double yr = (double)Yr[i], yi = (double)Yi[i];
+        if constexpr (COMP) yr += (double)Ylr[i], yi += (double)Yli[i];
+        const double o_r = qr * yr - qi * yi, o_i = qr * yi + qi * yr;
+        if constexpr (PREC == DYF_FP64) {
+            gptr_w<cx<double>> o = (gptr_w<cx<double>>)a.out + row + x;
+            o->r = o_r, o->i = o_i;
+        } else {
+            gptr_w<cx<float>> o = (gptr_w<cx<float>>)a.out + row + x;
+            o->r = (float)o_r, o->i = (float)o_i;
+        }
+    }
+}
+
+// Steps per loop iteration of the in-place march: four where that fits
+// two waves per SIMD without spilling inside the step loop, else two.
+template <int C>
+constexpr int inplace_unroll() {
+    return C == 13 || C == 18 ? 2 : 4;
+}
+
+template <int C, int PREC>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+void laue_fast_inplace_kernel(FastArgs a) {
+    const int L = logical_block(a.per_xcd);
+    if (L >= a.total) return;
+    const int r = L / a.n_e, ie = L - r * a.n_e;
+    fast_row_inplace<C, PREC, inplace_unroll<C>()>(a, r, ie);
 }
 
 template <int C, int MODE, int PREC>
@@ -786,7 +1022,9 @@ template <int MODE, int PREC, int... Cs>
 struct FastTable {
     static FastFn get(int C) {
         FastFn f = nullptr;
-        if constexpr (MODE == DYF_MODE_GRID && PREC != DYF_FP64)
+        if constexpr (MODE == DYF_MODE_GRID && PREC == DYF_FP64)
+            ((C == Cs ? (f = laue_fast_inplace_kernel<Cs, PREC>, 0) : 0), ...);
+        else if constexpr (MODE == DYF_MODE_GRID)
             ((C == Cs ? (f = laue_fast_unrolled_kernel<Cs, PREC>, 0) : 0), ...);
         else
             ((C == Cs ? (f = laue_fast_kernel<Cs, MODE, PREC>, 0) : 0), ...);

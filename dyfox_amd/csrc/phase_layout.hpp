@@ -62,10 +62,17 @@ inline int phase_layout(const dyf_laue_args *a, int mode, int prec,
     // The interaction-picture march divides by the diagonal propagator;
     // keep it well conditioned: |sigma_h| ds small (always true on exact
     // grids at sub-micron pitch).
+    // The grid-mode march addresses one row's table with 32-bit byte
+    // offsets (a buffer resource per row): nsteps * C * 64 entries must
+    // stay below 2^31 bytes (Nz up to ~116,000 at C = 18 in FP64).
     const double sh = std::sqrt(a->shr * a->shr + a->shi * a->shi);
-    const bool fast = a->shift_px == 1.0 && L->C > 0 && sh * a->ds < 0.25;
-    L->path = fast ? DYF_PATH_FAST : DYF_PATH_GENERAL;
     const size_t elem = prec == DYF_FP64 ? 16 : 8;
+    const bool row_fits =
+        (size_t)L->nsteps * (size_t)(L->C > 0 ? L->C : 1) * kWave * elem <
+        ((size_t)1 << 31);
+    const bool fast =
+        a->shift_px == 1.0 && L->C > 0 && sh * a->ds < 0.25 && row_fits;
+    L->path = fast ? DYF_PATH_FAST : DYF_PATH_GENERAL;
     L->bytes = 0;
     if (fast && mode == DYF_MODE_GRID)
         L->bytes = (size_t)L->nrows * L->nsteps * L->C * kWave * elem;
