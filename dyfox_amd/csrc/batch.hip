@@ -132,6 +132,7 @@ extern "C" int dyf_laue_batch_prepare(const dyf_laue_instance *in, int n,
         d.n_e = I.n_e, d.nrows = l.L.nrows, d.iy_lo = l.L.iy_lo;
         d.nsteps = l.L.nsteps, d.Nx = I.grid.Nx, d.Ny = I.grid.Ny, d.C = l.L.C;
         d.pad_ = l.nseg_kept;
+        d.fan_lo = l.L.fan_lo, d.fan_hi = l.L.fan_hi;
         d.dx = I.grid.dx, d.sheet_hw = I.grid.sheet_hw;
         d.n_items = (long long)l.L.nrows * I.n_e;
     }

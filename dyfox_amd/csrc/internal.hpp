@@ -34,6 +34,7 @@ struct InstDesc {
     const void *coef;
     void *out;
     int n_e, nrows, iy_lo, nsteps, Nx, Ny, C, pad_;
+    int fan_lo, fan_hi;   // columns the incident beam lights
     double dx, sheet_hw;
     long long item_base, n_items;
 };

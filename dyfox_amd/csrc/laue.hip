@@ -156,6 +156,7 @@ struct FastArgs {
     const double *qN;
     void *out;           // (n_e, Ny, Nx) complex
     int n_e, nrows, iy_lo, nsteps, Nx, Ny, per_xcd, total;
+    int fan_lo, fan_hi;  // columns the incident beam lights (PhaseLayout)
     double dx, sheet_hw;
 };
 
@@ -194,9 +195,13 @@ __device__ __forceinline__ T fmaT(T a, T b, T c) {
         return __builtin_fmaf(a, b, c);
 }
 
-// March one (row r, angle ie) with the calling wavefront (64 lanes).
+// March one (row r, angle ie) with the calling wavefront (64 lanes):
+// the analytic mode (one e^{iH} per row and step, folded into the step's
+// coefficients) and the pristine mode.  Grid-mode marches, which stream a
+// per-column e^{iH} table, are fast_row_inplace and fast_row_unrolled.
 template <int C, int MODE, int PREC>
 __device__ __forceinline__ void fast_row(const FastArgs &a, int r, int ie) {
+    static_assert(MODE != DYF_MODE_GRID, "grid mode has its own marches");
     using S = StateT<PREC>;
     using T = typename S::T;
     constexpr bool COMP = S::comp;
@@ -206,46 +211,21 @@ __device__ __forceinline__ void fast_row(const FastArgs &a, int r, int ie) {
     // fields: hi parts (and compensations for MIXED)
     T Xr[C], Xi[C], Yr[C], Yi[C];
     T Xlr[COMP ? C : 1], Xli[COMP ? C : 1], Ylr[COMP ? C : 1], Yli[COMP ? C : 1];
-    T msk[MODE == DYF_MODE_GRID ? 1 : C];
+    T msk[C];
 #pragma unroll
     for (int i = 0; i < C; ++i) {
         const int x = lane * C + i;
         Xr[i] = x < a.Nx ? (T)beam_profile(x, a.Nx, a.dx, a.sheet_hw) : T(0);
         Xi[i] = Yr[i] = Yi[i] = T(0);
         if constexpr (COMP) Xlr[i] = Xli[i] = Ylr[i] = Yli[i] = T(0);
-        if constexpr (MODE != DYF_MODE_GRID) msk[i] = x < a.Nx ? T(1) : T(0);
+        msk[i] = x < a.Nx ? T(1) : T(0);
     }
 
-    const cx<T> *E = nullptr;
-    if constexpr (MODE == DYF_MODE_GRID)
-        E = (const cx<T> *)a.phase + (size_t)r * a.nsteps * (C * kWave) + lane;
     const cx<T> *Erow = nullptr;
     if constexpr (MODE == DYF_MODE_ANALYTIC)
         Erow = (const cx<T> *)a.phase + (size_t)r * a.nsteps;
 
-    // One register set for the e^{iH} slice, prefetched on a rolling
-    // basis: column i's value for step k+1 is loaded into Ec[i] as soon
-    // as column i of step k has used it, so every load has a whole step
-    // to land and no second slice of registers is held (C = 17 FP64: 68
-    // VGPRs fewer, a third wave per SIMD).
-    // In FP32 the first slice is loaded in column order, as the loop
-    // reloads it: the wait counts at the loop head are then exact
-    // (vmcnt(C - 1)) instead of a full drain.
-    cx<T> Ec[MODE == DYF_MODE_GRID ? C : 1];
-    if constexpr (MODE == DYF_MODE_GRID) {
-#pragma unroll
-        for (int i = 0; i < C; ++i) {
-            Ec[i] = E[i * kWave];
-            if constexpr (sizeof(T) == 4) __builtin_amdgcn_sched_barrier(0);
-        }
-    }
-
     for (int k = 0; k < a.nsteps; ++k) {
-        const cx<T> *Ek = nullptr;
-        if constexpr (MODE == DYF_MODE_GRID) {
-            const int kn = k + 1 < a.nsteps ? k + 1 : k;
-            Ek = E + (size_t)kn * (C * kWave);
-        }
         T cr, ci, br, bi;
         const size_t ci_ = (size_t)__builtin_amdgcn_readfirstlane(ie) * a.nsteps + k;
         if constexpr (sizeof(T) == 8) {
@@ -279,25 +259,10 @@ __device__ __forceinline__ void fast_row(const FastArgs &a, int r, int ie) {
             const T xi = i + 1 < C ? Xi[i + 1] : xin_i;
             const T yr = i > 0 ? Yr[i - 1] : yin_r;
             const T yi = i > 0 ? Yi[i - 1] : yin_i;
-            T tr, ti, ur, ui;   // t = e^{iH} y,  u = e^{-iH} x
-            if constexpr (MODE == DYF_MODE_GRID) {
-                const T er = Ec[i].r, ei = Ec[i].i;
-                tr = fmaT(er, yr, -ei * yi);
-                ti = fmaT(er, yi, ei * yr);
-                ur = fmaT(er, xr, ei * xi);
-                ui = fmaT(er, xi, -ei * xr);
-                // the next step's value, into the register just read (the
-                // fence keeps the load from being hoisted into a copy).  In
-                // FP32 a scheduling barrier keeps it from sinking to the end
-                // of the step, so it has a whole step to land (mixed march
-                // -2 %; in FP64 the barrier cost 1 %)
-                asm volatile("" ::: "memory");
-                Ec[i] = Ek[i * kWave];
-                if constexpr (sizeof(T) == 4) __builtin_amdgcn_sched_barrier(0);
-            } else {
-                tr = msk[i] * yr, ti = msk[i] * yi;
-                ur = msk[i] * xr, ui = msk[i] * xi;
-            }
+            // the row-uniform e^{iH} is in the coefficients; the mask
+            // keeps the padding columns from coupling back
+            const T tr = msk[i] * yr, ti = msk[i] * yi;
+            const T ur = msk[i] * xr, ui = msk[i] * xi;
             if constexpr (!COMP) {
                 nXr[i] = fmaT(cr, tr, fmaT(-ci, ti, xr));
                 nXi[i] = fmaT(cr, ti, fmaT(ci, tr, xi));
@@ -371,6 +336,10 @@ using gptr_w = T __attribute__((address_space(1))) *;
 // Buffer-resource word 3 for raw (unswizzled, stride 0) buffers on gfx9
 // parts: 32-bit data format, no index stride.
 constexpr int kBufferWord3 = 0x00020000;
+// A lane's byte offset for the slices it does not need (before its first
+// step in the beam's fan): past every row's range (rows stay below 2^31
+// bytes), so the buffer load returns zeros without reading memory.
+constexpr int kOutOfRange = 0x7FFFFFF0;
 
 template <typename T>
 __device__ __forceinline__ cx<T> load_cx(__amdgpu_buffer_rsrc_t rs, int voff,
@@ -413,7 +382,15 @@ __device__ __forceinline__ void fast_row_unrolled(const FastArgs &a, int r,
         (void *)((const cx<T> *)a.phase + (size_t)r * a.nsteps * kSlice), 0,
         a.nsteps * kSlice * kEl, kBufferWord3);
     const int loff = lane * kEl;
-    auto slice = [&](int k, int i) { return load_cx<T>(rs, loff, (k * C + i) * kWave * kEl); };
+    // ROLL (the stand-alone march): slices before the lane's first step
+    // in the beam's fan are not read.  The persistent kernel reads them
+    // (the table holds zeros there): the per-step selection of its loads'
+    // offsets spills registers at C = 17 and 18.
+    const int kfirst = lane_first_step(lane, C, a.fan_lo, a.fan_hi);
+    auto slice = [&](int k, int i) {
+        return load_cx<T>(rs, (!ROLL || k >= kfirst) ? loff : kOutOfRange,
+                          (k * C + i) * kWave * kEl);
+    };
 
     // ROLL: the e^{iH} slice is prefetched on a rolling basis, as in
     // fast_row (column i's value for step k+1 is loaded as soon as column
@@ -576,7 +553,13 @@ __device__ __forceinline__ void fast_row_inplace(const FastArgs &a, int r,
         (void *)((const cx<T> *)a.phase + (size_t)r * a.nsteps * kSlice), 0,
         a.nsteps * kSlice * kEl, kBufferWord3);
     const int loff = lane * kEl;
-    auto slice = [&](int k, int i) { return load_cx<T>(rs, loff, (k * C + i) * kWave * kEl); };
+    // slices before the lane's first step in the beam's fan are not read
+    // (the fields there are exact zeros)
+    const int kfirst = lane_first_step(lane, C, a.fan_lo, a.fan_hi);
+    auto slice = [&](int k, int i) {
+        return load_cx<T>(rs, k >= kfirst ? loff : kOutOfRange,
+                          (k * C + i) * kWave * kEl);
+    };
     // column i's e^{iH}, or between the two passes the product the second
     // pass needs
     T Sr[C], Si[C];
@@ -948,6 +931,7 @@ void laue_persistent_kernel(
             fa.n_e = d.n_e, fa.nrows = d.nrows, fa.iy_lo = d.iy_lo;
             fa.nsteps = d.nsteps, fa.Nx = d.Nx, fa.Ny = d.Ny;
             fa.per_xcd = 0, fa.total = 0;
+            fa.fan_lo = d.fan_lo, fa.fan_hi = d.fan_hi;
             fa.dx = d.dx, fa.sheet_hw = d.sheet_hw;
             fast_row_unrolled<C, PREC>(fa, r, ie);
         }
@@ -1154,6 +1138,8 @@ extern "C" int dyf_laue_march(const dyf_laue_args *a, int mode, int prec,
         fa.Ny = a->Ny;
         fa.total = nrows * n_e;
         fa.per_xcd = (fa.total + 7) / 8;
+        fa.fan_lo = L.fan_lo;
+        fa.fan_hi = L.fan_hi;
         fa.dx = a->dx;
         fa.sheet_hw = a->sheet_hw;
         FastFn fn = pick_fast(mode, prec, L.C);

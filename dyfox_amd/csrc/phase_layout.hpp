@@ -24,8 +24,44 @@ struct PhaseLayout {
     int C;          // columns per lane (fast path)
     int iy_lo, iy_hi, nrows;
     int nsteps;     // Nz - 1
+    int fan_lo, fan_hi;   // columns where the incident beam can be non-zero
     size_t bytes;   // phase buffer size for (mode, prec)
 };
+
+// Columns [lo, hi] where the march's incident profile (the top-hat
+// |(x - Nx/2) dx| < sheet_hw, or the Gaussian of sigma -sheet_hw) can be
+// non-zero, evaluated with the device's own expression.  A Gaussian is
+// exactly zero once (x/sigma)^2 / 2 > 745 (exp underflows); 800 is used.
+// A degenerate profile (sheet_hw = 0, or no column lit) gives the whole
+// row.
+inline void beam_support(const dyf_laue_args *a, int *lo, int *hi) {
+    int l = a->Nx, h = -1;
+    const double hw = a->sheet_hw;
+    for (int x = 0; x < a->Nx; ++x) {
+        const double xp = (double)(x - a->Nx / 2) * a->dx;
+        const bool lit = hw > 0.0 ? std::fabs(xp) < hw
+                                  : xp * xp < 1600.0 * hw * hw;
+        if (lit) l = x < l ? x : l, h = x > h ? x : h;
+    }
+    if (hw == 0.0 || l > h) l = 0, h = a->Nx - 1;
+    *lo = l, *hi = h;
+}
+
+// The march's fields start non-zero only on columns [fan_lo, fan_hi] and
+// spread one column per step each way (X moves left, Y right), so before
+// step k they vanish outside [fan_lo - k, fan_hi + k] and the step reads
+// e^{iH} only on [fan_lo - k - 1, fan_hi + k + 1]; outside, it multiplies
+// exact zeros.  Lane `lane` (columns [lane C, lane C + C)) needs its
+// e^{iH} from this step on (one column of margin each way); the table
+// producers skip the entries before it and the marches do not load them.
+__host__ __device__ inline int lane_first_step(int lane, int C, int fan_lo,
+                                               int fan_hi) {
+    const int x0 = lane * C, x1 = x0 + C - 1;
+    const int kl = fan_lo - 2 - x1;   // lanes left of the fan
+    const int kr = x0 - fan_hi - 2;   // lanes right of it
+    const int k = kl > kr ? kl : kr;
+    return k > 0 ? k : 0;
+}
 
 inline int fast_cols_for(int Nx) {
     const int need = (Nx + kWave - 1) / kWave;
@@ -59,6 +95,7 @@ inline int phase_layout(const dyf_laue_args *a, int mode, int prec,
     L->nrows = band.iy_hi - band.iy_lo;
     L->nsteps = a->Nz - 1;
     L->C = fast_cols_for(a->Nx);
+    beam_support(a, &L->fan_lo, &L->fan_hi);
     // The interaction-picture march divides by the diagonal propagator;
     // keep it well conditioned: |sigma_h| ds small (always true on exact
     // grids at sub-micron pitch).

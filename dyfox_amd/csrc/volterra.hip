@@ -88,13 +88,15 @@ __global__ __launch_bounds__(256) void volterra_grid_kernel(
 }
 
 // ---- fused e^{iH} table for the fast march ---------------------------
-// Thread t of a (row, step block) covers x = lane*C + i with i = t / 64,
-// lane = t % 64, for kPhaseZ consecutive steps, and writes element t of
-// each of those (row, step) blocks.  Columns past Nx (the lane padding)
-// are written as zeros.  The grid is flat over (row, step block, t): a
-// tile's C * 64 threads are whole wavefronts, so no wavefront is idle
-// (with 256-thread blocks per tile, 1088 = 17 x 64 threads left three of
-// every twenty wavefronts without work).
+// Thread t of a (row, step block) covers column x = t (0 <= x < C * 64,
+// lane x / C, slot x % C) for kPhaseZ consecutive steps and writes that
+// column's element of each of those (row, step) blocks.  A wavefront
+// covers 64 consecutive columns, so the wavefronts wholly outside the
+// beam's fan skip their evaluation as a unit: entries of a lane before its
+// first step (lane_first_step), which the march multiplies by exact zeros,
+// are written as zeros, as are the columns past Nx (the lane padding).
+// The grid is flat over (row, step block, t): a tile's C * 64 threads are
+// whole wavefronts, so no wavefront is idle.
 constexpr int kPhaseZ = 4;
 
 static dim3 phase_table_grid(const PhaseLayout &L) {
@@ -106,8 +108,8 @@ static dim3 phase_table_grid(const PhaseLayout &L) {
 template <typename T>
 __device__ __forceinline__ void phase_table_points(
     cx<T> *__restrict__ E, const SegParams *__restrict__ segs, int nseg,
-    int C, int Nx, int Ny, int nsteps, int nrows, int iy_lo, double dx,
-    double dz, double a2) {
+    int C, int Nx, int Ny, int nsteps, int nrows, int iy_lo, int fan_lo,
+    int fan_hi, double dx, double dz, double a2) {
     __shared__ LogEntry tab[kLogTab];
     load_log_table(tab);
     __syncthreads();
@@ -115,15 +117,15 @@ __device__ __forceinline__ void phase_table_points(
     const size_t g = (size_t)blockIdx.x * 256 + threadIdx.x;
     const size_t tile = g / per;
     if (tile >= (size_t)nrows * nzb) return;
-    const int t = (int)(g - tile * per);
+    const int x = (int)(g - tile * per);
     const int r = (int)(tile / nzb);
     const int k0 = (int)(tile - (size_t)r * nzb) * kPhaseZ;
-    const int i = t / kWave, lane = t % kWave;
-    const int x = lane * C + i;
-    const size_t stride = (size_t)C * kWave;
-    cx<T> *out = E + ((size_t)r * nsteps + k0) * stride + t;
+    const int lane = x / C, i = x - lane * C;
     const int nk = min(kPhaseZ, nsteps - k0);
-    if (x >= Nx) {
+    const size_t stride = (size_t)C * kWave;
+    cx<T> *out = E + ((size_t)r * nsteps + k0) * stride + (size_t)i * kWave + lane;
+    // padding, or every step of this block before the lane's first step
+    if (x >= Nx || k0 + nk - 1 < lane_first_step(lane, C, fan_lo, fan_hi)) {
 #pragma unroll
         for (int j = 0; j < kPhaseZ; ++j)
             if (j < nk) out[j * stride] = cx<T>{T(0), T(0)};
@@ -150,9 +152,10 @@ __device__ __forceinline__ void phase_table_points(
 template <typename T>
 __global__ __launch_bounds__(256) void phase_from_segments_kernel(
     cx<T> *__restrict__ E, int nseg, int C, int Nx, int Ny, int nsteps,
-    int nrows, int iy_lo, double dx, double dz, double a2) {
+    int nrows, int iy_lo, int fan_lo, int fan_hi, double dx, double dz,
+    double a2) {
     phase_table_points<T>(E, c_segs, nseg, C, Nx, Ny, nsteps, nrows, iy_lo,
-                          dx, dz, a2);
+                          fan_lo, fan_hi, dx, dz, a2);
 }
 
 // Same table with the segments read from device memory (uniform loads),
@@ -161,10 +164,10 @@ __global__ __launch_bounds__(256) void phase_from_segments_kernel(
 template <typename T>
 __global__ __launch_bounds__(256) void phase_segments_dev_kernel(
     cx<T> *__restrict__ E, const SegParams *__restrict__ segs, int nseg,
-    int C, int Nx, int Ny, int nsteps, int nrows, int iy_lo, double dx,
-    double dz, double a2) {
-    phase_table_points<T>(E, segs, nseg, C, Nx, Ny, nsteps, nrows, iy_lo, dx,
-                          dz, a2);
+    int C, int Nx, int Ny, int nsteps, int nrows, int iy_lo, int fan_lo,
+    int fan_hi, double dx, double dz, double a2) {
+    phase_table_points<T>(E, segs, nseg, C, Nx, Ny, nsteps, nrows, iy_lo,
+                          fan_lo, fan_hi, dx, dz, a2);
 }
 
 int launch_phase_segments_dev(const dyf_laue_args *a, const PhaseLayout &L,
@@ -175,11 +178,11 @@ int launch_phase_segments_dev(const dyf_laue_args *a, const PhaseLayout &L,
     if (prec == DYF_FP64)
         phase_segments_dev_kernel<double><<<grid, 256, 0, stream>>>(
             (cx<double> *)phase, segs, nseg, L.C, a->Nx, a->Ny, L.nsteps,
-            L.nrows, L.iy_lo, a->dx, a->dz, a2);
+            L.nrows, L.iy_lo, L.fan_lo, L.fan_hi, a->dx, a->dz, a2);
     else
         phase_segments_dev_kernel<float><<<grid, 256, 0, stream>>>(
             (cx<float> *)phase, segs, nseg, L.C, a->Nx, a->Ny, L.nsteps,
-            L.nrows, L.iy_lo, a->dx, a->dz, a2);
+            L.nrows, L.iy_lo, L.fan_lo, L.fan_hi, a->dx, a->dz, a2);
     DYF_LAUNCHED("phase_segments_dev_kernel");
     return DYF_OK;
 }
@@ -339,11 +342,11 @@ extern "C" int dyf_laue_phase_from_segments(const dyf_laue_args *a, int prec,
     if (prec == DYF_FP64)
         phase_from_segments_kernel<double><<<grid, 256, 0, stream>>>(
             (cx<double> *)phase, kept, L.C, a->Nx, a->Ny, L.nsteps, L.nrows,
-            L.iy_lo, a->dx, a->dz, a2);
+            L.iy_lo, L.fan_lo, L.fan_hi, a->dx, a->dz, a2);
     else
         phase_from_segments_kernel<float><<<grid, 256, 0, stream>>>(
             (cx<float> *)phase, kept, L.C, a->Nx, a->Ny, L.nsteps, L.nrows,
-            L.iy_lo, a->dx, a->dz, a2);
+            L.iy_lo, L.fan_lo, L.fan_hi, a->dx, a->dz, a2);
     DYF_LAUNCHED("phase_from_segments_kernel");
     return DYF_OK;
 }

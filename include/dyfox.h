@@ -132,7 +132,11 @@ int dyf_laue_coef_bytes(const dyf_laue_args *a, int n_e, size_t *bytes);
 int dyf_laue_phase_from_h(const dyf_laue_args *a, int prec, const void *H_dev,
                           int H_prec, void *phase_dev, void *stream);
 /* Fast path: e^{iH} table evaluated from segments with no H grid stored
- * (FP64 Volterra, 2 pi reduction in FP64 for the FP32/mixed table).       */
+ * (FP64 Volterra, 2 pi reduction in FP64 for the FP32/mixed table).
+ * Entries outside the Borrmann fan of a->sheet_hw's incident beam, where
+ * the march's fields are exact zeros, hold 0 instead of e^{iH}: the
+ * table is for marches with the same beam (dyf_laue_march skips those
+ * entries whatever they hold).                                          */
 int dyf_laue_phase_from_segments(const dyf_laue_args *a, int prec,
                                  const double *segs, int nseg,
                                  const double *g_vec, double nu,

@@ -316,3 +316,40 @@ def test_odd_row_count_agrees_across_paths(dev, diamond400, line):
     even = LaueGrid(diamond400, 40e-6, 0.25e-6, 8e-6, Ny=34)
     other = to_numpy(solve_dislocation(seg, s, even, diamond400))
     assert np.abs(other[:, 1:33] - fused[:, 0:32]).max() > 1e-6
+
+
+@pytest.mark.parametrize("beam", ["sheet", "gauss_narrow", "gauss_wide",
+                                  "sheet_wide"])
+def test_beam_fan_agrees_across_paths(dev, diamond400, rng, beam):
+    """The fused e^{iH} table is evaluated only inside the Borrmann fan of
+    the incident beam (zeros outside, where the march multiplies exact
+    zeros), and the marches read no slice before a lane's first step in
+    the fan.  A narrow top-hat sheet, a Gaussian whose exactly-zero tails
+    leave part of the row dark, a Gaussian wider than the row and a wide
+    sheet: the fused solve, the H_gpu= path (a full table) and the oracle
+    agree in FP64, and mixed precision stays within 1e-6."""
+    kw = {"sheet": dict(beam_thickness=1e-6),
+          "gauss_narrow": dict(beam_sigma=0.5e-6),
+          "gauss_wide": dict(beam_sigma=20e-6),
+          "sheet_wide": dict(beam_thickness=30e-6)}[beam]
+    grid = LaueGrid(diamond400, 40e-6, 0.25e-6, 4e-6, **kw)
+    segs = random_segments(rng, diamond400, grid, 3)
+    s = _s(diamond400, [-90.0, 0.0, 117.5])
+    want = oracle_exit(segs, s, grid, diamond400)
+    fused = to_numpy(solve_dislocation(segs, s, grid, diamond400))
+    H = compute_H_grid(segs, grid, diamond400)
+    via_h = to_numpy(solve_dislocation(segs, s, grid, diamond400, H_gpu=H))
+    _check(fused, want, 1e-10, f"{beam} fused")
+    _check(via_h, want, 1e-10, f"{beam} H_gpu")
+    mixed = to_numpy(solve_dislocation(segs, s, grid, diamond400,
+                                       precision=MIXED))
+    _check(mixed, want, 1e-6, f"{beam} mixed")
+    if beam == "sheet":
+        # lit columns Nx/2 - 1 .. Nx/2 + 1 spread one column per step: the
+        # exit wave is exactly zero outside, on the device as in the oracle
+        Nx = grid.shape[1]
+        lo, hi = Nx // 2 - 1 - (grid.Nz - 1), Nx // 2 + 1 + (grid.Nz - 1)
+        assert lo > 0 and hi < Nx - 1
+        assert not np.any(fused[..., :lo]) and not np.any(fused[..., hi + 1:])
+        assert not np.any(want[..., :lo]) and not np.any(want[..., hi + 1:])
+
