@@ -313,16 +313,15 @@ __device__ __forceinline__ void fast_row(const FastArgs &a, int r, int ie) {
     }
 }
 
-// fast_row for the persistent batched march (GRID mode only): each step's
-// e^{iH} slice is loaded at the start of its step and the step loop is
-// unrolled by C.  The shift moves every column one register over per
-// step, so after C steps the register assignment is back where it
-// started and the allocator needs no copies for the loop-carried state
-// (C = 17 FP64: 289 VALU instructions per step instead of 339; mixed 489
-// instead of 587).  In the persistent kernel (capped at two waves per
-// SIMD) this measured faster (256-instance ensemble, mixed: 4.80 s -> 4.10 s); in the
-// stand-alone kernel the same source measured 5-7 % slower than the
-// rolled fast_row above, so the two are kept apart.
+// The grid-mode march with FP32 state (stand-alone, ROLL) and the
+// persistent batched march with FP32 state: the step loop is unrolled by
+// C.  The shift moves every column one register over per step, so after
+// C steps the register assignment is back where it started and the
+// allocator needs no copies for the loop-carried state (C = 17 mixed: 489
+// VALU instructions per step instead of 587).  Without ROLL each step's
+// e^{iH} slice is loaded at the start of its step (the persistent kernel,
+// capped at two waves per SIMD: 256-instance ensemble, mixed 4.80 s ->
+// 4.10 s when this replaced the rolled loop there).
 // The persistent kernel reads its table, coefficient and output pointers
 // from descriptors in device memory, so the compiler cannot tell they are
 // global and would access them as flat memory, whose loads also count
@@ -382,14 +381,16 @@ __device__ __forceinline__ void fast_row_unrolled(const FastArgs &a, int r,
         (void *)((const cx<T> *)a.phase + (size_t)r * a.nsteps * kSlice), 0,
         a.nsteps * kSlice * kEl, kBufferWord3);
     const int loff = lane * kEl;
-    // ROLL (the stand-alone march): slices before the lane's first step
-    // in the beam's fan are not read.  The persistent kernel reads them
-    // (the table holds zeros there): the per-step selection of its loads'
-    // offsets spills registers at C = 17 and 18.
+    // Slices before the lane's first step in the beam's fan are not read
+    // (the fields there are exact zeros), except at C = 18 without ROLL,
+    // where selecting the offset spills registers inside the step loop
+    // (the table holds zeros there, so reading them is harmless).
     const int kfirst = lane_first_step(lane, C, a.fan_lo, a.fan_hi);
+    auto slice_at = [&](int voff, int k, int i) {
+        return load_cx<T>(rs, voff, (k * C + i) * kWave * kEl);
+    };
     auto slice = [&](int k, int i) {
-        return load_cx<T>(rs, (!ROLL || k >= kfirst) ? loff : kOutOfRange,
-                          (k * C + i) * kWave * kEl);
+        return slice_at(k >= kfirst ? loff : kOutOfRange, k, i);
     };
 
     // ROLL: the e^{iH} slice is prefetched on a rolling basis, as in
@@ -406,9 +407,16 @@ __device__ __forceinline__ void fast_row_unrolled(const FastArgs &a, int r,
 
     auto step = [&](int k) {
         const int kn = ROLL ? (k + 1 < a.nsteps ? k + 1 : k) : k;
-        if constexpr (!ROLL) {
+        if constexpr (!ROLL && C >= 18) {
 #pragma unroll
-            for (int i = 0; i < C; ++i) Ec[i] = slice(k, i);
+            for (int i = 0; i < C; ++i) Ec[i] = slice_at(loff, k, i);
+        } else if constexpr (!ROLL) {
+            // the offset chosen once per step (the empty asm keeps the
+            // compiler from re-deriving it for each load)
+            int voff = k >= kfirst ? loff : kOutOfRange;
+            asm volatile("" : "+v"(voff));
+#pragma unroll
+            for (int i = 0; i < C; ++i) Ec[i] = slice_at(voff, k, i);
         }
         T cr, ci, br, bi;
         const size_t ci_ = (size_t)__builtin_amdgcn_readfirstlane(ie) * a.nsteps + k;
@@ -933,7 +941,13 @@ void laue_persistent_kernel(
             fa.per_xcd = 0, fa.total = 0;
             fa.fan_lo = d.fan_lo, fa.fan_hi = d.fan_hi;
             fa.dx = d.dx, fa.sheet_hw = d.sheet_hw;
-            fast_row_unrolled<C, PREC>(fa, r, ie);
+            // FP64: the in-place march (no register copies, slices skipped
+            // before a lane's first step in the fan); FP32 state: the
+            // step loop unrolled by C
+            if constexpr (PREC == DYF_FP64)
+                fast_row_inplace<C, PREC, inplace_unroll<C>()>(fa, r, ie);
+            else
+                fast_row_unrolled<C, PREC>(fa, r, ie);
         }
     }
 }
