@@ -125,6 +125,8 @@ _SIGNATURES = {
                              C.POINTER(C.c_size_t)],
     "dyf_laue_coef_bytes": [C.POINTER(LaueArgs), C.c_int,
                             C.POINTER(C.c_size_t)],
+    "dyf_laue_march_fraction": [C.POINTER(LaueArgs), C.c_int, C.c_int,
+                                C.c_int, C.POINTER(C.c_double)],
     "dyf_laue_phase_from_h": [C.POINTER(LaueArgs), C.c_int, vp, C.c_int, vp,
                               vp],
     "dyf_laue_phase_from_segments": [C.POINTER(LaueArgs), C.c_int, dp,

@@ -738,6 +738,327 @@ void laue_fast_inplace_kernel(FastArgs a) {
     fast_row_inplace<C, PREC, inplace_unroll<C>()>(a, r, ie);
 }
 
+// ---- two angles per wavefront while the beam's fan is narrow ------------
+// Before step K every non-zero field value, and every e^{iH} a step
+// reads, lies on the 32 lanes [g0, g0 + 32) (the fan argument of
+// lane_first_step; the host picks g0 and the largest such K).  Until then
+// one wavefront marches two angles of the same row, one per half: lane l
+// stands for lane g0 + (l & 31) of the one-angle layout, both halves read
+// the same slices, each half takes its own angle's coefficients, and a
+// half's edge lanes take zero inflow (the lanes beyond them hold exact
+// zeros).  At step K the second angle's state goes to LDS, the first
+// angle's moves to its one-angle lanes, and each angle finishes on the
+// full wavefront in turn.  Every value is formed by the same operations
+// as in fast_row_inplace (results equal up to the sign of zeros outside
+// the fan); a row costs 2 nsteps - K wavefront-steps instead of 2 nsteps.
+struct PairGeom {
+    int gl;              // lane of the one-angle layout this lane stands for
+    int kfirst;          // first step the lane needs its e^{iH}
+    bool ledge, redge;   // no inflow from the left / right neighbour
+};
+
+template <typename T>
+__device__ __forceinline__ T pair_take(T v, int src) {
+    if constexpr (sizeof(T) == 8) {
+        const long long b = __double_as_longlong(v);
+        const int lo = __builtin_amdgcn_ds_bpermute(src << 2, (int)b);
+        const int hi = __builtin_amdgcn_ds_bpermute(src << 2, (int)(b >> 32));
+        return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+    } else {
+        return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(v)));
+    }
+}
+// value of the right / left neighbour lane, 0 at an edge
+template <typename T>
+__device__ __forceinline__ T pair_right(T v, int lane, bool edge) {
+    const T r = pair_take(v, lane + 1);
+    return edge ? T(0) : r;
+}
+template <typename T>
+__device__ __forceinline__ T pair_left(T v, int lane, bool edge) {
+    const T r = pair_take(v, lane - 1);
+    return edge ? T(0) : r;
+}
+
+template <int C, int PREC, int U>
+__device__ __forceinline__ void fast_row_pair(const FastArgs &a, int r,
+                                              int ie0, int ie1, int K,
+                                              int g0, char *lds_raw) {
+    using S = StateT<PREC>;
+    using T = typename S::T;
+    constexpr bool COMP = S::comp;
+    constexpr int NF = COMP ? 8 : 4;          // fields held per column
+    constexpr int kSlice = C * kWave;
+    constexpr int kEl = (int)sizeof(cx<T>);
+    const int lane = threadIdx.x & (kWave - 1);
+    const int half = lane >> 5, hl = lane & 31;
+    const int iy = a.iy_lo + r;
+    T *lds = (T *)lds_raw;
+
+    // state: field f of column i is F[f][i]: X re, X im, Y re, Y im, then
+    // (mixed) their compensations
+    T F[NF][C], Sr[C], Si[C];
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)((const cx<T> *)a.phase + (size_t)r * a.nsteps * kSlice), 0,
+        a.nsteps * kSlice * kEl, kBufferWord3);
+    auto load_slice = [&](const PairGeom &g, int k) {
+        const int voff = k >= g.kfirst ? g.gl * kEl : kOutOfRange;
+#pragma unroll
+        for (int i = 0; i < C; ++i) {
+            const cx<T> e = load_cx<T>(rs, voff, (k * C + i) * kWave * kEl);
+            Sr[i] = e.r, Si[i] = e.i;
+        }
+    };
+    auto scalar_coefs = [&](int ie, int k, T &cr, T &ci, T &br, T &bi) {
+        const size_t at = (size_t)ie * a.nsteps + k;
+        if constexpr (sizeof(T) == 8) {
+            kptr<CoefD> c = (kptr<CoefD>)a.cd + at;
+            cr = c->cr, ci = c->ci, br = c->br, bi = c->bi;
+        } else {
+            kptr<CoefF> c = (kptr<CoefF>)a.cf + at;
+            cr = c->cr, ci = c->ci, br = c->br, bi = c->bi;
+        }
+    };
+    // X'[i] = X[i+1] + c t, Y'[i] = Y[i-1] + b u, in place (with the
+    // compensated split of fast_row_inplace in mixed precision)
+    auto new_x = [&](int i, T xr, T xi, T xlr, T xli, T tr, T ti, T cr, T ci) {
+        if constexpr (!COMP) {
+            F[0][i] = fmaT(cr, tr, fmaT(-ci, ti, xr));
+            F[1][i] = fmaT(cr, ti, fmaT(ci, tr, xi));
+        } else {
+            const T dr = fmaT(cr, tr, fmaT(-ci, ti, xlr));
+            const T di = fmaT(cr, ti, fmaT(ci, tr, xli));
+            const T sr = xr + dr, si = xi + di;
+            F[4][i] = dr - (sr - xr), F[5][i] = di - (si - xi);
+            F[0][i] = sr, F[1][i] = si;
+        }
+    };
+    auto new_y = [&](int i, T yr, T yi, T ylr, T yli, T ur, T ui, T br, T bi) {
+        if constexpr (!COMP) {
+            F[2][i] = fmaT(br, ur, fmaT(-bi, ui, yr));
+            F[3][i] = fmaT(br, ui, fmaT(bi, ur, yi));
+        } else {
+            const T dr = fmaT(br, ur, fmaT(-bi, ui, ylr));
+            const T di = fmaT(br, ui, fmaT(bi, ur, yli));
+            const T sr = yr + dr, si = yi + di;
+            F[6][i] = dr - (sr - yr), F[7][i] = di - (si - yi);
+            F[2][i] = sr, F[3][i] = si;
+        }
+    };
+
+    // steps [kb, ke) on geometry g; coefs(k, cr, ci, br, bi)
+    auto march = [&](int kb, int ke, const PairGeom &g, auto coefs) {
+        auto step = [&](int k, auto xfirst) {
+            constexpr bool XFIRST = decltype(xfirst)::value;
+            T cr, ci, br, bi;
+            coefs(k, cr, ci, br, bi);
+            const int kn = k + 1 < ke ? k + 1 : k;
+            const int voff = kn >= g.kfirst ? g.gl * kEl : kOutOfRange;
+            // inflow across lane boundaries (old values)
+            T xin[COMP ? 4 : 2], yin[COMP ? 4 : 2];
+            xin[0] = pair_right(F[0][0], lane, g.redge);
+            xin[1] = pair_right(F[1][0], lane, g.redge);
+            yin[0] = pair_left(F[2][C - 1], lane, g.ledge);
+            yin[1] = pair_left(F[3][C - 1], lane, g.ledge);
+            if constexpr (COMP) {
+                xin[2] = pair_right(F[4][0], lane, g.redge);
+                xin[3] = pair_right(F[5][0], lane, g.redge);
+                yin[2] = pair_left(F[6][C - 1], lane, g.ledge);
+                yin[3] = pair_left(F[7][C - 1], lane, g.ledge);
+            }
+            // sources of column i: X from x+1, Y from x-1
+            auto xs = [&](int i, T &vr, T &vi, T &lr, T &li) {
+                vr = i + 1 < C ? F[0][i + 1] : xin[0];
+                vi = i + 1 < C ? F[1][i + 1] : xin[1];
+                if constexpr (COMP) {
+                    lr = i + 1 < C ? F[4][i + 1] : xin[2];
+                    li = i + 1 < C ? F[5][i + 1] : xin[3];
+                }
+            };
+            auto ys = [&](int i, T &vr, T &vi, T &lr, T &li) {
+                vr = i > 0 ? F[2][i - 1] : yin[0];
+                vi = i > 0 ? F[3][i - 1] : yin[1];
+                if constexpr (COMP) {
+                    lr = i > 0 ? F[6][i - 1] : yin[2];
+                    li = i > 0 ? F[7][i - 1] : yin[3];
+                }
+            };
+            auto first = [&](int i) {
+                T xr, xi, xlr = T(0), xli = T(0), yr, yi, ylr = T(0), yli = T(0);
+                xs(i, xr, xi, xlr, xli);
+                ys(i, yr, yi, ylr, yli);
+                const T er = Sr[i], ei = Si[i];
+                const T tr = fmaT(er, yr, -ei * yi);    // t = e^{iH} y
+                const T ti = fmaT(er, yi, ei * yr);
+                const T ur = fmaT(er, xr, ei * xi);     // u = e^{-iH} x
+                const T ui = fmaT(er, xi, -ei * xr);
+                if constexpr (XFIRST) {
+                    new_x(i, xr, xi, xlr, xli, tr, ti, cr, ci);
+                    Sr[i] = ur, Si[i] = ui;
+                } else {
+                    new_y(i, yr, yi, ylr, yli, ur, ui, br, bi);
+                    Sr[i] = tr, Si[i] = ti;
+                }
+            };
+            auto second = [&](int i) {
+                if constexpr (XFIRST) {
+                    T yr, yi, ylr = T(0), yli = T(0);
+                    ys(i, yr, yi, ylr, yli);
+                    new_y(i, yr, yi, ylr, yli, Sr[i], Si[i], br, bi);
+                } else {
+                    T xr, xi, xlr = T(0), xli = T(0);
+                    xs(i, xr, xi, xlr, xli);
+                    new_x(i, xr, xi, xlr, xli, Sr[i], Si[i], cr, ci);
+                }
+                asm volatile("" ::: "memory");
+                const cx<T> e = load_cx<T>(rs, voff, (kn * C + i) * kWave * kEl);
+                Sr[i] = e.r, Si[i] = e.i;
+            };
+            if constexpr (XFIRST) {
+#pragma unroll
+                for (int i = 0; i < C; ++i) first(i);
+#pragma unroll
+                for (int i = C - 1; i >= 0; --i) second(i);
+            } else {
+#pragma unroll
+                for (int i = C - 1; i >= 0; --i) first(i);
+#pragma unroll
+                for (int i = 0; i < C; ++i) second(i);
+            }
+        };
+        using XF = std::integral_constant<bool, true>;
+        using YF = std::integral_constant<bool, false>;
+        int k = kb;
+        if constexpr (U > 2) {
+            for (; k + U <= ke; k += U) {
+#pragma unroll
+                for (int kk = 0; kk < U; kk += 2) {
+                    step(k + kk, XF{});
+                    step(k + kk + 1, YF{});
+                }
+            }
+        }
+        for (; k + 2 <= ke; k += 2) {
+            step(k, XF{});
+            step(k + 1, YF{});
+        }
+        if (k < ke) step(k, XF{});
+    };
+    auto write_out = [&](int ie) {
+        gptr<double> qN = (gptr<double>)a.qN;
+        const double qr = qN[2 * ie], qi = qN[2 * ie + 1];
+        const size_t row = ((size_t)ie * a.Ny + iy) * a.Nx;
+#pragma unroll
+        for (int i = 0; i < C; ++i) {
+            const int x = lane * C + i;
+            if (x >= a.Nx) continue;
+            double yr = (double)F[2][i], yi = (double)F[3][i];
+            if constexpr (COMP) yr += (double)F[6][i], yi += (double)F[7][i];
+            const double o_r = qr * yr - qi * yi, o_i = qr * yi + qi * yr;
+            if constexpr (PREC == DYF_FP64) {
+                gptr_w<cx<double>> o = (gptr_w<cx<double>>)a.out + row + x;
+                o->r = o_r, o->i = o_i;
+            } else {
+                gptr_w<cx<float>> o = (gptr_w<cx<float>>)a.out + row + x;
+                o->r = (float)o_r, o->i = (float)o_i;
+            }
+        }
+    };
+
+    // phase 1: both angles, one per half
+    PairGeom g;
+    g.gl = g0 + hl;
+    g.kfirst = lane_first_step(g.gl, C, a.fan_lo, a.fan_hi);
+    g.ledge = hl == 0;
+    g.redge = hl == 31;
+#pragma unroll
+    for (int i = 0; i < C; ++i) {
+        const int x = g.gl * C + i;
+        F[0][i] = x < a.Nx ? (T)beam_profile(x, a.Nx, a.dx, a.sheet_hw) : T(0);
+#pragma unroll
+        for (int f = 1; f < NF; ++f) F[f][i] = T(0);
+    }
+    load_slice(g, 0);
+    const int s0 = __builtin_amdgcn_readfirstlane(ie0);
+    const int s1 = __builtin_amdgcn_readfirstlane(ie1);
+    march(0, K, g, [&](int k, T &cr, T &ci, T &br, T &bi) {
+        T c0r, c0i, b0r, b0i, c1r, c1i, b1r, b1i;
+        scalar_coefs(s0, k, c0r, c0i, b0r, b0i);
+        scalar_coefs(s1, k, c1r, c1i, b1r, b1i);
+        cr = half ? c1r : c0r, ci = half ? c1i : c0i;
+        br = half ? b1r : b0r, bi = half ? b1i : b0i;
+    });
+
+    // the second angle's state to LDS ([field][column][32 lanes])
+    if (half) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+#pragma unroll
+            for (int i = 0; i < C; ++i) lds[(f * C + i) * 32 + hl] = F[f][i];
+    }
+    // the first angle's state to its one-angle lanes
+    {
+        const int j = lane - g0;
+        const bool inwin = j >= 0 && j < 32;
+        const int src = inwin ? j : 0;
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+#pragma unroll
+            for (int i = 0; i < C; ++i) {
+                const T v = pair_take(F[f][i], src);
+                F[f][i] = inwin ? v : T(0);
+            }
+    }
+    __syncthreads();
+
+    // phase 2: each angle on the full wavefront
+    g.gl = lane;
+    g.kfirst = lane_first_step(lane, C, a.fan_lo, a.fan_hi);
+    g.ledge = lane == 0;
+    g.redge = lane == kWave - 1;
+    // (no slice past the row's last step: the buffer's range check covers
+    // the lane offset, not the scalar one)
+    if (K < a.nsteps) load_slice(g, K);
+    march(K, a.nsteps, g, [&](int k, T &cr, T &ci, T &br, T &bi) {
+        scalar_coefs(s0, k, cr, ci, br, bi);
+    });
+    write_out(ie0);
+    if (ie1 == ie0) return;
+    {
+        const int j = lane - g0;
+        const bool inwin = j >= 0 && j < 32;
+        const int src = inwin ? j : 0;
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+#pragma unroll
+            for (int i = 0; i < C; ++i)
+                F[f][i] = inwin ? lds[(f * C + i) * 32 + src] : T(0);
+    }
+    if (K < a.nsteps) load_slice(g, K);
+    march(K, a.nsteps, g, [&](int k, T &cr, T &ci, T &br, T &bi) {
+        scalar_coefs(s1, k, cr, ci, br, bi);
+    });
+    write_out(ie1);
+}
+
+// One wavefront per (row, pair of angles); LDS holds the second angle's
+// state between the phases (4 fields x C x 32 lanes in FP64, 8 in FP32
+// for the mixed state: 1 KB per column either way).  ie1 = ie0 for the
+// last pair of an odd count (its second half marches a copy, not
+// written).
+template <int C, int PREC>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+void laue_fast_pair_kernel(FastArgs a, int K, int g0) {
+    extern __shared__ double pair_lds[];
+    const int L = logical_block(a.per_xcd);
+    if (L >= a.total) return;
+    const int npair = (a.n_e + 1) / 2;
+    const int r = L / npair, p = L - r * npair;
+    const int ie0 = 2 * p, ie1 = 2 * p + 1 < a.n_e ? 2 * p + 1 : 2 * p;
+    // two steps per loop iteration: four spill inside the loops at C = 17
+    fast_row_pair<C, PREC, 2>(a, r, ie0, ie1, K, g0, (char *)pair_lds);
+}
+
 template <int C, int MODE, int PREC>
 __global__ __launch_bounds__(64) void laue_fast_kernel(FastArgs a) {
     const int L = logical_block(a.per_xcd);
@@ -1035,6 +1356,52 @@ static FastFn fast_fn(int C) {
     return FastTable<MODE, PREC, 2, 4, 6, 8, 10, 12, 13, 14, 16, 17, 18>::get(C);
 }
 
+using PairFn = void (*)(FastArgs, int, int);
+
+template <int PREC, int... Cs>
+static PairFn pair_fn(int C) {
+    PairFn f = nullptr;
+    ((C == Cs ? (f = laue_fast_pair_kernel<Cs, PREC>, 0) : 0), ...);
+    return f;
+}
+
+// The pair march's window [g0, g0 + 32) and its step K: the largest K with
+// every lane outside the window still before its first step in the fan.
+static void pair_window(const PhaseLayout &L, int *K, int *g0) {
+    int bestK = 0, bestg = 0;
+    for (int g = 0; g <= kWave / 2; ++g) {
+        int k = L.nsteps;
+        for (int l = 0; l < kWave; ++l)
+            if (l < g || l >= g + kWave / 2)
+                k = std::min(k, lane_first_step(l, L.C, L.fan_lo, L.fan_hi));
+        if (k > bestK) bestK = k, bestg = g;
+    }
+    *K = bestK, *g0 = bestg;
+}
+
+// Whether a fast-path march pairs angles, and its (K, g0) if so (K = 0:
+// one angle per wavefront).  Grid mode, two angles or more, while the fan
+// fits half a wavefront for at least an eighth of the steps, and when the
+// pairs still fill the device twice over at two waves per SIMD (halving
+// the wavefronts of a small launch costs more than the pairing saves: the
+// 2-angle worked example, 0.46 -> 0.55 ms); not at C = 18, where the pair
+// march spills registers.
+static int pair_plan(const PhaseLayout &L, int mode, int n_e, int *K,
+                     int *g0) {
+    *K = 0, *g0 = 0;
+    if (L.path != DYF_PATH_FAST || mode != DYF_MODE_GRID || n_e < 2 ||
+        L.C >= 18 || L.nrows == 0)
+        return DYF_OK;
+    int dev = 0, cus = 256;
+    DYF_HIP(hipGetDevice(&dev));
+    DYF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if ((long long)L.nrows * ((n_e + 1) / 2) < 16LL * cus) return DYF_OK;
+    int k = 0, g = 0;
+    pair_window(L, &k, &g);
+    if (k >= 1 && 8 * k >= L.nsteps) *K = k, *g0 = g;
+    return DYF_OK;
+}
+
 static FastFn pick_fast(int mode, int prec, int C) {
 #define DYF_PICK(M, P) \
     if (mode == M && prec == P) return fast_fn<M, P>(C);
@@ -1086,6 +1453,23 @@ extern "C" int dyf_laue_coef_bytes(const dyf_laue_args *a, int n_e,
     return DYF_OK;
 }
 
+extern "C" int dyf_laue_march_fraction(const dyf_laue_args *a, int mode,
+                                       int prec, int n_e, double *fraction) {
+    PhaseLayout L;
+    int rc = phase_layout(a, mode, prec, &L);
+    if (rc) return rc;
+    if (n_e < 0 || !fraction) return fail(DYF_EINVAL, "bad n_e or fraction");
+    *fraction = 1.0;
+    int K = 0, g0 = 0;
+    if ((rc = pair_plan(L, mode, n_e, &K, &g0))) return rc;
+    if (K == 0 || n_e == 0) return DYF_OK;
+    // wavefront-steps of the paired launch over those of one angle each
+    const double pairs = (double)(n_e / 2), ns = (double)L.nsteps;
+    const double odd = (n_e % 2) ? ns : 0.0;
+    *fraction = (pairs * (2.0 * ns - K) + odd) / ((double)n_e * ns);
+    return DYF_OK;
+}
+
 extern "C" int dyf_laue_march(const dyf_laue_args *a, int mode, int prec,
                               const void *src, int H_prec,
                               const dyf_line_params *line,
@@ -1121,6 +1505,9 @@ extern "C" int dyf_laue_march(const dyf_laue_args *a, int mode, int prec,
                         "but the device allows 160 KB; reduce the x-grid",
                         lds / 1024);
     }
+    // the march's form is settled before the first launch too
+    int K = 0, g0 = 0;
+    if (int prc = pair_plan(L, mode, n_e, &K, &g0)) return prc;
     const int elem = prec == DYF_FP64 ? 16 : 8;
     const int nrows = L.nrows;
     if (nrows < a->Ny) {
@@ -1156,6 +1543,23 @@ extern "C" int dyf_laue_march(const dyf_laue_args *a, int mode, int prec,
         fa.fan_hi = L.fan_hi;
         fa.dx = a->dx;
         fa.sheet_hw = a->sheet_hw;
+        if (K > 0) {
+            PairFn pf = nullptr;
+            if (prec == DYF_FP64)
+                pf = pair_fn<DYF_FP64, 2, 4, 6, 8, 10, 12, 13, 14, 16, 17>(L.C);
+            else if (prec == DYF_FP32)
+                pf = pair_fn<DYF_FP32, 2, 4, 6, 8, 10, 12, 13, 14, 16, 17>(L.C);
+            else
+                pf = pair_fn<DYF_MIXED, 2, 4, 6, 8, 10, 12, 13, 14, 16, 17>(L.C);
+            if (!pf) return fail(DYF_EKEY, "no pair march for C=%d", L.C);
+            fa.total = nrows * ((n_e + 1) / 2);
+            fa.per_xcd = (fa.total + 7) / 8;
+            const size_t lds = (size_t)L.C * 1024;   // 4 doubles or 8 floats per lane
+            hipLaunchKernelGGL(pf, dim3(8 * fa.per_xcd), dim3(kWave), lds,
+                               stream, fa, K, g0);
+            DYF_LAUNCHED("laue_fast_pair_kernel");
+            return DYF_OK;
+        }
         FastFn fn = pick_fast(mode, prec, L.C);
         hipLaunchKernelGGL(fn, dim3(8 * fa.per_xcd), dim3(kWave), 0, stream, fa);
         DYF_LAUNCHED("laue_fast_kernel");

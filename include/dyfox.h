@@ -155,6 +155,13 @@ int dyf_laue_march(const dyf_laue_args *a, int mode, int prec,
                    const void *src_dev, int H_prec,
                    const dyf_line_params *line, const double *s_dev_dev,
                    int n_e, void *coef_dev, void *out_dev, void *stream);
+/* The fraction of the launch's pixel-steps (rows x Nx x (Nz-1) x n_e)
+ * that dyf_laue_march executes: below 1 when it marches two angles per
+ * wavefront while the beam's fan fits half a wavefront (the rest are
+ * exact zeros it does not compute), else 1.  No counterpart in the
+ * reference (which computes every pixel-step); for roofline accounting. */
+int dyf_laue_march_fraction(const dyf_laue_args *a, int mode, int prec,
+                            int n_e, double *fraction);
 /* dyf_laue_march without the coefficient workspace: the same kernels and
  * results, with the per-(angle, step) table in a library-held device array
  * of dyf_laue_march_noalloc_max_bytes() (4 MiB: n_e x (Nz-1) up to 87,381)

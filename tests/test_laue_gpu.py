@@ -353,3 +353,28 @@ def test_beam_fan_agrees_across_paths(dev, diamond400, rng, beam):
         assert not np.any(fused[..., :lo]) and not np.any(fused[..., hi + 1:])
         assert not np.any(want[..., :lo]) and not np.any(want[..., hi + 1:])
 
+
+@pytest.mark.parametrize("prec", [FP64, MIXED])
+@pytest.mark.parametrize("n_e", [128, 127])
+def test_pair_march_equals_single_angle_march(dev, diamond400, rng, prec,
+                                              n_e):
+    """A launch with enough (row, angle pair) wavefronts to fill the device
+    twice marches two angles per wavefront while the beam's fan fits half
+    a wavefront (81 rows x 64 pairs here); a one-angle launch never pairs.
+    Every angle of the paired launch, the unpaired last one of an odd count
+    included, equals its own one-angle solve exactly (np.array_equal: up to
+    the sign of zeros outside the fan), and the one-angle march is the one
+    checked against the oracle above."""
+    grid = LaueGrid(diamond400, 80e-6, 0.25e-6, 20e-6)
+    segs = random_segments(rng, diamond400, grid, 3)
+    s = _s(diamond400, np.linspace(-200.0, 200.0, n_e))
+    full = to_numpy(solve_dislocation(segs, s, grid, diamond400,
+                                      precision=prec))
+    for j in (0, 1, n_e // 2, n_e - 2, n_e - 1):
+        one = to_numpy(solve_dislocation(segs, s[j:j + 1], grid, diamond400,
+                                         precision=prec))
+        assert np.array_equal(full[j], one[0]), (prec.name, n_e, j)
+    if prec is FP64:
+        want = oracle_exit(segs, s[[0, n_e - 1]], grid, diamond400)
+        _check(full[[0, n_e - 1]], want, 1e-10, f"pair{n_e}")
+
