@@ -15,6 +15,12 @@
 
 namespace dyf {
 
+// Work items of an instance's row in the persistent march: angle pairs
+// where the pair march runs (C <= pair_max_c(prec)), else angles.
+static long long batch_units(int prec, int C, int n_e) {
+    return C <= pair_max_c(prec) ? (n_e + 1) / 2 : n_e;
+}
+
 static size_t al256(size_t n) { return (n + 255) & ~(size_t)255; }
 
 struct InstLayout {
@@ -63,7 +69,7 @@ static int layout_batch(const dyf_laue_instance *in, int n, int prec,
             if ((*out)[i].L.C == b.C) {
                 (*out)[i].bucket = (int)buckets->size();
                 (*out)[i].slot = b.n++;
-                b.items += (long long)(*out)[i].L.nrows * in[i].n_e;
+                b.items += (long long)(*out)[i].L.nrows * batch_units(prec, b.C, in[i].n_e);
             }
         if (b.n == 0) continue;
         b.desc_off = m;
@@ -133,8 +139,17 @@ extern "C" int dyf_laue_batch_prepare(const dyf_laue_instance *in, int n,
         d.nsteps = l.L.nsteps, d.Nx = I.grid.Nx, d.Ny = I.grid.Ny, d.C = l.L.C;
         d.pad_ = l.nseg_kept;
         d.fan_lo = l.L.fan_lo, d.fan_hi = l.L.fan_hi;
+        // pair items: two angles per wavefront while the fan fits half a
+        // wavefront, for at least an eighth of the steps (else K = 0: the
+        // pair's angles one after the other)
+        d.pair_K = 0, d.pair_g0 = 0;
+        if (l.L.C <= pair_max_c(prec) && I.n_e >= 2) {
+            int K = 0, g0 = 0;
+            laue_pair_window(l.L, &K, &g0);
+            if (K >= 1 && 8 * K >= l.L.nsteps) d.pair_K = K, d.pair_g0 = g0;
+        }
         d.dx = I.grid.dx, d.sheet_hw = I.grid.sheet_hw;
-        d.n_items = (long long)l.L.nrows * I.n_e;
+        d.n_items = (long long)l.L.nrows * batch_units(prec, l.L.C, I.n_e);
     }
     // item bases follow slot order inside each bucket
     for (size_t b = 0; b < bk.size(); ++b) {

@@ -35,11 +35,25 @@ struct InstDesc {
     void *out;
     int n_e, nrows, iy_lo, nsteps, Nx, Ny, C, pad_;
     int fan_lo, fan_hi;   // columns the incident beam lights
+    int pair_K, pair_g0;  // pair march (C <= pair_max_c): steps with two angles per
+                          // wavefront and their lane window (K = 0: none)
     double dx, sheet_hw;
     long long item_base, n_items;
 };
 
+// The pair march's lane window [g0, g0 + 32) for a layout and its step K:
+// the largest K before which every lane outside the window is still
+// before its first step in the beam's fan.
+void laue_pair_window(const PhaseLayout &L, int *K, int *g0);
+
+// The largest column count whose persistent march takes (row, pair of
+// angles) items: beyond it the pair march spills registers inside its
+// step loops (FP64 at C = 17, every precision at C = 18).
+constexpr int pair_max_c(int prec) { return prec == DYF_FP64 ? 16 : 17; }
+
 // Persistent march over `n_inst` instances that share column count C: every wave takes (row, angle)
+// items (C <= pair_max_c(prec): (row, pair of angles) items, marched as
+// fast_row_pair does)
 // items from 8 XCD-local queues (stealing when its own is empty) until
 // all are drained.  counters: 8 unsigned ints, zeroed by this call.
 int launch_laue_persistent(const InstDesc *inst_dev, int n_inst,

@@ -965,39 +965,43 @@ __device__ __forceinline__ void fast_row_pair(const FastArgs &a, int r,
         }
     };
 
-    // phase 1: both angles, one per half
-    PairGeom g;
-    g.gl = g0 + hl;
-    g.kfirst = lane_first_step(g.gl, C, a.fan_lo, a.fan_hi);
-    g.ledge = hl == 0;
-    g.redge = hl == 31;
+    // the incident beam on geometry g
+    auto init_state = [&](const PairGeom &g) {
 #pragma unroll
-    for (int i = 0; i < C; ++i) {
-        const int x = g.gl * C + i;
-        F[0][i] = x < a.Nx ? (T)beam_profile(x, a.Nx, a.dx, a.sheet_hw) : T(0);
+        for (int i = 0; i < C; ++i) {
+            const int x = g.gl * C + i;
+            F[0][i] = x < a.Nx ? (T)beam_profile(x, a.Nx, a.dx, a.sheet_hw) : T(0);
 #pragma unroll
-        for (int f = 1; f < NF; ++f) F[f][i] = T(0);
-    }
-    load_slice(g, 0);
+            for (int f = 1; f < NF; ++f) F[f][i] = T(0);
+        }
+    };
     const int s0 = __builtin_amdgcn_readfirstlane(ie0);
     const int s1 = __builtin_amdgcn_readfirstlane(ie1);
-    march(0, K, g, [&](int k, T &cr, T &ci, T &br, T &bi) {
-        T c0r, c0i, b0r, b0i, c1r, c1i, b1r, b1i;
-        scalar_coefs(s0, k, c0r, c0i, b0r, b0i);
-        scalar_coefs(s1, k, c1r, c1i, b1r, b1i);
-        cr = half ? c1r : c0r, ci = half ? c1i : c0i;
-        br = half ? b1r : b0r, bi = half ? b1i : b0i;
-    });
+    PairGeom g;
+    if (K > 0) {
+        // phase 1: both angles, one per half
+        g.gl = g0 + hl;
+        g.kfirst = lane_first_step(g.gl, C, a.fan_lo, a.fan_hi);
+        g.ledge = hl == 0;
+        g.redge = hl == 31;
+        init_state(g);
+        load_slice(g, 0);
+        march(0, K, g, [&](int k, T &cr, T &ci, T &br, T &bi) {
+            T c0r, c0i, b0r, b0i, c1r, c1i, b1r, b1i;
+            scalar_coefs(s0, k, c0r, c0i, b0r, b0i);
+            scalar_coefs(s1, k, c1r, c1i, b1r, b1i);
+            cr = half ? c1r : c0r, ci = half ? c1i : c0i;
+            br = half ? b1r : b0r, bi = half ? b1i : b0i;
+        });
 
-    // the second angle's state to LDS ([field][column][32 lanes])
-    if (half) {
+        // the second angle's state to LDS ([field][column][32 lanes])
+        if (half) {
 #pragma unroll
-        for (int f = 0; f < NF; ++f)
+            for (int f = 0; f < NF; ++f)
 #pragma unroll
-            for (int i = 0; i < C; ++i) lds[(f * C + i) * 32 + hl] = F[f][i];
-    }
-    // the first angle's state to its one-angle lanes
-    {
+                for (int i = 0; i < C; ++i) lds[(f * C + i) * 32 + hl] = F[f][i];
+        }
+        // the first angle's state to its one-angle lanes
         const int j = lane - g0;
         const bool inwin = j >= 0 && j < 32;
         const int src = inwin ? j : 0;
@@ -1008,14 +1012,16 @@ __device__ __forceinline__ void fast_row_pair(const FastArgs &a, int r,
                 const T v = pair_take(F[f][i], src);
                 F[f][i] = inwin ? v : T(0);
             }
+        __syncthreads();
     }
-    __syncthreads();
 
-    // phase 2: each angle on the full wavefront
+    // phase 2: each angle on the full wavefront (K = 0: the whole march,
+    // one angle after the other)
     g.gl = lane;
     g.kfirst = lane_first_step(lane, C, a.fan_lo, a.fan_hi);
     g.ledge = lane == 0;
     g.redge = lane == kWave - 1;
+    if (K == 0) init_state(g);
     // (no slice past the row's last step: the buffer's range check covers
     // the lane offset, not the scalar one)
     if (K < a.nsteps) load_slice(g, K);
@@ -1024,7 +1030,7 @@ __device__ __forceinline__ void fast_row_pair(const FastArgs &a, int r,
     });
     write_out(ie0);
     if (ie1 == ie0) return;
-    {
+    if (K > 0) {
         const int j = lane - g0;
         const bool inwin = j >= 0 && j < 32;
         const int src = inwin ? j : 0;
@@ -1033,6 +1039,8 @@ __device__ __forceinline__ void fast_row_pair(const FastArgs &a, int r,
 #pragma unroll
             for (int i = 0; i < C; ++i)
                 F[f][i] = inwin ? lds[(f * C + i) * 32 + src] : T(0);
+    } else {
+        init_state(g);
     }
     if (K < a.nsteps) load_slice(g, K);
     march(K, a.nsteps, g, [&](int k, T &cr, T &ci, T &br, T &bi) {
@@ -1232,6 +1240,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 void laue_persistent_kernel(
     const InstDesc *__restrict__ inst, int n_inst, long long total,
     long long per_queue, unsigned *__restrict__ counters) {
+    extern __shared__ double pers_lds[];   // pair items: 1 KB x C per wave
     const int lane = threadIdx.x & (kWave - 1);
     const int q0 = blockIdx.x & 7;   // block b runs on XCD b % 8
     for (int qq = 0; qq < 8; ++qq) {
@@ -1251,7 +1260,6 @@ void laue_persistent_kernel(
             }
             const InstDesc &d = inst[a];
             const long long local = item - d.item_base;
-            const int r = (int)(local / d.n_e), ie = (int)(local % d.n_e);
             FastArgs fa;
             fa.phase = d.phase;
             const CoefPtrs cp = coef_ptrs((void *)d.coef, d.n_e, d.nsteps);
@@ -1262,13 +1270,24 @@ void laue_persistent_kernel(
             fa.per_xcd = 0, fa.total = 0;
             fa.fan_lo = d.fan_lo, fa.fan_hi = d.fan_hi;
             fa.dx = d.dx, fa.sheet_hw = d.sheet_hw;
-            // FP64: the in-place march (no register copies, slices skipped
-            // before a lane's first step in the fan); FP32 state: the
-            // step loop unrolled by C
-            if constexpr (PREC == DYF_FP64)
-                fast_row_inplace<C, PREC, inplace_unroll<C>()>(fa, r, ie);
-            else
-                fast_row_unrolled<C, PREC>(fa, r, ie);
+            if constexpr (C <= pair_max_c(PREC)) {
+                // (row, pair of angles): two angles per wavefront for the
+                // first pair_K steps, then one after the other
+                const int npair = (d.n_e + 1) / 2;
+                const int r = (int)(local / npair), p = (int)(local % npair);
+                const int ie0 = 2 * p, ie1 = 2 * p + 1 < d.n_e ? 2 * p + 1 : 2 * p;
+                char *lds = (char *)pers_lds + (size_t)(threadIdx.x / kWave) * 1024 * C;
+                fast_row_pair<C, PREC, 2>(fa, r, ie0, ie1, d.pair_K, d.pair_g0, lds);
+            } else {
+                const int r = (int)(local / d.n_e), ie = (int)(local % d.n_e);
+                // FP64: the in-place march (no register copies, slices
+                // skipped before a lane's first step in the fan); FP32
+                // state: the step loop unrolled by C
+                if constexpr (PREC == DYF_FP64)
+                    fast_row_inplace<C, PREC, inplace_unroll<C>()>(fa, r, ie);
+                else
+                    fast_row_unrolled<C, PREC>(fa, r, ie);
+            }
         }
     }
 }
@@ -1324,11 +1343,17 @@ int launch_laue_persistent(const InstDesc *inst, int n_inst, long long total,
     else
         fn = persist_fn<DYF_MIXED, 2, 4, 6, 8, 10, 12, 13, 14, 16, 17, 18>(C);
     if (!fn) return fail(DYF_EKEY, "no persistent kernel for C=%d", C);
+    // the pair march keeps one angle's state in LDS: 1 KB per column per wave
+    const size_t lds = C <= pair_max_c(prec) ? (size_t)wpb * 1024 * C : 0;
+    if (lds > 65536)
+        DYF_HIP(hipFuncSetAttribute((const void *)fn,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds));
     zero_counters_kernel<<<1, 64, 0, stream>>>(counters);
     DYF_LAUNCHED("zero_counters_kernel");
     const long long per_queue = (total + 7) / 8;
     if (per_queue > 0xffffffffLL) return fail(DYF_EINVAL, "too many items");
-    hipLaunchKernelGGL(fn, dim3(blocks), dim3(kWave * wpb), 0, stream, inst,
+    hipLaunchKernelGGL(fn, dim3(blocks), dim3(kWave * wpb), lds, stream, inst,
                        n_inst, total, per_queue, counters);
     DYF_LAUNCHED("laue_persistent_kernel");
     return DYF_OK;
@@ -1400,6 +1425,10 @@ static int pair_plan(const PhaseLayout &L, int mode, int n_e, int *K,
     pair_window(L, &k, &g);
     if (k >= 1 && 8 * k >= L.nsteps) *K = k, *g0 = g;
     return DYF_OK;
+}
+
+void laue_pair_window(const PhaseLayout &L, int *K, int *g0) {
+    pair_window(L, K, g0);
 }
 
 static FastFn pick_fast(int mode, int prec, int C) {
