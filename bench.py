@@ -640,6 +640,8 @@ def main():
         device()
         r = time_ensembles()
         for name, v in r.items():
+            if not isinstance(v, dict):
+                continue
             print(f"probe-ensemble {name}: {v['seconds']:.3f} s, "
                   f"{v['value']:.3e} pixel-steps/s", flush=True)
         return None
