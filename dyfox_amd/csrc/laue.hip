@@ -780,7 +780,10 @@ __device__ __forceinline__ T pair_left(T v, int lane, bool edge) {
     return edge ? T(0) : r;
 }
 
-template <int C, int PREC, int U>
+// ANY_K: K may be 0 (the persistent march); the stand-alone kernel is
+// launched with K > 0 only, and compiling without the K = 0 paths keeps
+// its first phase free of register spills.
+template <int C, int PREC, int U, bool ANY_K>
 __device__ __forceinline__ void fast_row_pair(const FastArgs &a, int r,
                                               int ie0, int ie1, int K,
                                               int g0, char *lds_raw) {
@@ -978,7 +981,7 @@ __device__ __forceinline__ void fast_row_pair(const FastArgs &a, int r,
     const int s0 = __builtin_amdgcn_readfirstlane(ie0);
     const int s1 = __builtin_amdgcn_readfirstlane(ie1);
     PairGeom g;
-    if (K > 0) {
+    if (!ANY_K || K > 0) {
         // phase 1: both angles, one per half
         g.gl = g0 + hl;
         g.kfirst = lane_first_step(g.gl, C, a.fan_lo, a.fan_hi);
@@ -1021,7 +1024,8 @@ __device__ __forceinline__ void fast_row_pair(const FastArgs &a, int r,
     g.kfirst = lane_first_step(lane, C, a.fan_lo, a.fan_hi);
     g.ledge = lane == 0;
     g.redge = lane == kWave - 1;
-    if (K == 0) init_state(g);
+    if constexpr (ANY_K)
+        if (K == 0) init_state(g);
     // (no slice past the row's last step: the buffer's range check covers
     // the lane offset, not the scalar one)
     if (K < a.nsteps) load_slice(g, K);
@@ -1030,7 +1034,7 @@ __device__ __forceinline__ void fast_row_pair(const FastArgs &a, int r,
     });
     write_out(ie0);
     if (ie1 == ie0) return;
-    if (K > 0) {
+    if (!ANY_K || K > 0) {
         const int j = lane - g0;
         const bool inwin = j >= 0 && j < 32;
         const int src = inwin ? j : 0;
@@ -1064,7 +1068,7 @@ void laue_fast_pair_kernel(FastArgs a, int K, int g0) {
     const int r = L / npair, p = L - r * npair;
     const int ie0 = 2 * p, ie1 = 2 * p + 1 < a.n_e ? 2 * p + 1 : 2 * p;
     // two steps per loop iteration: four spill inside the loops at C = 17
-    fast_row_pair<C, PREC, 2>(a, r, ie0, ie1, K, g0, (char *)pair_lds);
+    fast_row_pair<C, PREC, 2, false>(a, r, ie0, ie1, K, g0, (char *)pair_lds);
 }
 
 template <int C, int MODE, int PREC>
@@ -1277,7 +1281,7 @@ void laue_persistent_kernel(
                 const int r = (int)(local / npair), p = (int)(local % npair);
                 const int ie0 = 2 * p, ie1 = 2 * p + 1 < d.n_e ? 2 * p + 1 : 2 * p;
                 char *lds = (char *)pers_lds + (size_t)(threadIdx.x / kWave) * 1024 * C;
-                fast_row_pair<C, PREC, 2>(fa, r, ie0, ie1, d.pair_K, d.pair_g0, lds);
+                fast_row_pair<C, PREC, 2, true>(fa, r, ie0, ie1, d.pair_K, d.pair_g0, lds);
             } else {
                 const int r = (int)(local / d.n_e), ie = (int)(local % d.n_e);
                 // FP64: the in-place march (no register copies, slices
