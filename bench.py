@@ -37,7 +37,7 @@ sys.path.insert(0, ROOT)
 METRIC = ("Laue pixel-steps/s FP64 vs mixed @<=1e-6 rel err; "
           "Bragg solves/s cold vs warm")
 FLOPS_PER_PIXEL_STEP = 40        # 2x2 complex matvec 28 + two couplings 12
-ISSUED_FLOPS_PER_PIXEL_STEP = 32  # the kernel's 16 FP64 FMAs
+ISSUED_FLOPS_PER_PIXEL_STEP = 32  # 16 FP64 FMA/MUL slots, 2 flops each
 BRAGG_FLOPS_PER_PIXEL_STEP = 28   # the reference's _march_down step
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
@@ -314,8 +314,11 @@ def time_bragg_matvec(N, xtal, repeats=20):
             "avg_launch_ms": ms,
             "note": "28 algorithmic flops per pixel-step (the reference's "
                     "_march_down step); the interaction-picture kernel "
-                    "issues 8 FP64 FMAs per pixel-step plus the halo "
-                    "lanes' duplicate columns"}
+                    "issues 16 FP64 FMA/MUL instructions per column-step "
+                    "(12 FMA + 4 MUL: 28 flops, or 32 counting each "
+                    "instruction as one FMA slot of the peak), plus the "
+                    "halo lanes' duplicate columns (64 lanes march the "
+                    "work of 52)"}
 
 
 def cpu_info():
@@ -748,8 +751,11 @@ def main():
             "note": "FP64 vector-ALU bound (no MFMA: the march is not a "
                     "contraction). achieved counts the 40 algorithmic "
                     "flops per executed pixel-step; the interaction-picture "
-                    "kernel issues 16 FP64 FMAs (32 flops), issued_* counts "
-                    "those. executed_fraction: the share of the workload's "
+                    "kernel issues 16 FP64 FMA/MUL instructions per "
+                    "pixel-step (12 FMA + 4 MUL by the VALU counters); "
+                    "issued_* counts each as one FMA slot (2 flops, the "
+                    "unit of the peak), so issued_frac is the share of FP64 "
+                    "issue slots used. executed_fraction: the share of the workload's "
                     "pixel-steps the march computes (two angles per "
                     "wavefront while the beam's fan fits half of one; the "
                     "others are exact zeros)",
