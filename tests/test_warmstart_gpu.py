@@ -145,3 +145,72 @@ def test_joint_policy_evaluation_by_band(dev, diamond400):
         assert rep["all"][name]["converged_fraction"] == 1.0
         assert rep["all"][name]["worst_residual"] <= 1e-10
     assert set(rep["bands"]) <= {"20-35um", "35-50um"}
+
+
+def test_committed_network_on_thick_crystals(dev, diamond400):
+    """The committed network at the thick end of the 20-100 um range
+    (66 and 100 um, one Krylov space per plane): its prediction is
+    finite on the larger grids, cold and network-started solves report
+    convergence exactly when every plane's true residual meets the
+    acceptance level, and wherever both converge they reach the same
+    surface (the guess changes the path, not the answer)."""
+    import os
+    from dyfox_amd.bragg.geometry import BraggConfig, bragg_g_vector
+    from dyfox_amd.bragg.solver import solve_surface_fields
+    from dyfox_amd.warmstart import data
+    from dyfox_amd.warmstart.net import W0, WarmStartNet
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    w = np.load(os.path.join(root, "dyfox_amd", "warmstart",
+                             "weights_cfg5.npz"))
+    net = WarmStartNet(8, params=w["params"])
+    rng = np.random.default_rng(11)
+    ts = data.thickness_grid(diamond400, (20e-6, 100e-6), 8)
+    inst = []
+    for t in (ts[4], ts[7]):
+        # two strained instances per thickness (a draw may have no line)
+        drawn = data.draw_instances(12, rng, diamond400, max_seg=2,
+                                    t_values=np.array([t]))
+        inst += [d for d in drawn if d["segs"]][:2]
+    assert len(inst) == 4
+    ds = data.build_dataset(diamond400, inst, n_solve=0)
+    ds.scale = float(w["scale"])
+    pred = data.predict(net, ds, list(range(len(inst))))
+    assert np.isfinite(pred).all() and np.abs(pred).max() > 0
+    g = bragg_g_vector(diamond400)
+    ys = data.plane_ys()
+    tol, accept = 1e-10, 1e-7
+    n_both = 0
+    for grp in ([0, 1], [2, 3]):
+        cfg = BraggConfig(diamond400, t_crystal=inst[grp[0]]["t"],
+                          dx=data.DX, half_x=data.HALF_X)
+        assert cfg.Nz > 200                    # the thick end: 66, 100 um
+        segs = [inst[k]["segs"] for k in grp]
+        s = np.array([inst[k]["s"] for k in grp])
+        vp = np.array([data.perfect_v(diamond400, inst[k]["s"], cfg)
+                       for k in grp])
+        guess = np.broadcast_to(vp[:, None, None],
+                                (len(grp), data.NPLANES, cfg.Nx)).copy()
+        x0 = (cfg.Nx - W0) // 2
+        guess[:, :, x0:x0 + W0] += pred[grp]
+        cold, ci = solve_surface_fields(segs, s, cfg, ys, diamond400, g,
+                                        tol=tol, planes="independent")
+        warm, wi = solve_surface_fields(segs, s, cfg, ys, diamond400, g,
+                                        tol=tol, planes="independent",
+                                        warm_start=guess)
+        cold, warm = np.asarray(cold), np.asarray(warm)
+        for j in range(len(grp)):
+            for info in (ci[j], wi[j]):
+                # the reference's acceptance: every plane's true residual
+                # at or below accept (1e-7)
+                assert info["converged"] == (info["residual"] <= accept)
+            if ci[j]["converged"] and wi[j]["converged"]:
+                n_both += 1
+                worst = max(ci[j]["residual"], wi[j]["residual"])
+                bound = 1e-6 if worst <= tol else 1e-3
+                diff = np.abs(warm[j] - cold[j]).max()
+                print(f"t {inst[grp[j]]['t'] * 1e6:.1f} um: residuals "
+                      f"{ci[j]['residual']:.1e} / {wi[j]['residual']:.1e}, "
+                      f"its {ci[j]['iterations']} / {wi[j]['iterations']}, "
+                      f"surface diff {diff / np.abs(cold[j]).max():.1e}")
+                assert diff <= bound * np.abs(cold[j]).max()
+    assert n_both >= 1
