@@ -26,6 +26,9 @@ import numpy as np
 from . import _native as N
 from .backend import FP64, precision as _precision
 
+# Bytes of exit waves one incoherent_image launch may hold by default.
+STACK_BUDGET = 4 << 30
+
 
 def condenser_angles(w_D, reach=75.0):
     """Incidence angles for an incoherent condenser sum and their
@@ -89,11 +92,14 @@ def convergence_bandwidth_nodes(xtal, sig_psi, sig_delta, n_psi, psi_max,
 
 
 def incoherent_image(seg_list, grid, xtal, optics, s_dev, weights,
-                     d_arg=None, ell=0, *, batch=16, precision=FP64):
+                     d_arg=None, ell=0, *, batch=None, precision=FP64):
     """sum_n weights[n] I_n on the device, I_n the image of the exit wave
     at deviation s_dev[n] through the charge-`ell` pupil, walked by the
     chromatic argument d_arg[n] when given (Optics.chromatic).  Returns
-    the (Ny, Nx) float64 map as a DeviceArray."""
+    the (Ny, Nx) float64 map as a DeviceArray.  `batch` nodes are marched
+    per launch; by default every node in one launch while their exit
+    waves fit STACK_BUDGET bytes (a launch of few angles leaves the device
+    partly idle and cannot march two angles per wavefront)."""
     from .laue.solver import march, prepare_phase
     from .optics import image_from_spectrum, object_spectrum
     s_dev = np.asarray(s_dev, np.float64)
@@ -105,11 +111,14 @@ def incoherent_image(seg_list, grid, xtal, optics, s_dev, weights,
         if d_arg.shape != s_dev.shape:
             raise ValueError("d_arg must match s_dev")
     Nx, Ny = grid.geometry()[:2]
+    prec = _precision(precision)
+    if batch is None:
+        per_node = Ny * Nx * np.dtype(prec.complex).itemsize
+        batch = int(max(1, min(len(s_dev), STACK_BUDGET // per_node)))
     total = N.DeviceArray.zeros((Ny, Nx), np.float64)
     plain = N.DeviceArray.zeros((Ny, Nx), np.float64)
     # one e^{iH} table for every node (the field does not change), then
     # each batch of nodes is one march
-    prec = _precision(precision)
     src = prepare_phase(seg_list, grid, xtal, precision=prec, reuse=True)
     sd_all = N.DeviceArray.from_host(s_dev)
     for b0 in range(0, len(s_dev), batch):
