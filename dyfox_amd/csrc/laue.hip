@@ -1410,11 +1410,15 @@ static void pair_window(const PhaseLayout &L, int *K, int *g0) {
 
 // Whether a fast-path march pairs angles, and its (K, g0) if so (K = 0:
 // one angle per wavefront).  Grid mode, two angles or more, while the fan
-// fits half a wavefront for at least an eighth of the steps, and when the
-// pairs still fill the device twice over at two waves per SIMD (halving
-// the wavefronts of a small launch costs more than the pairing saves: the
-// 2-angle worked example, 0.46 -> 0.55 ms); not at C = 18, where the pair
-// march spills registers.
+// fits half a wavefront for at least an eighth of the steps, and when
+// there are at least 6 pairs per CU (1.5 pair wavefronts per SIMD):
+// halving the wavefronts of a smaller launch costs more than the pairing
+// saves.  Measured on the 273 um rocking-scan grid (256 rows, C = 17),
+// march with / without pairs: 8 angles (4 pairs per CU) FP64 1.01 / 0.83
+// ms, 12 angles (6 per CU) 1.15 / 1.17 ms (mixed 0.88 / 1.10), 16 angles
+// 1.39 / 1.48 (mixed 0.99 / 1.24), 24 angles 1.85 / 2.15; the 2-angle
+// worked example (481 pairs) 0.55 / 0.46 ms.  Not at C = 18, where the
+// pair march spills registers.
 static int pair_plan(const PhaseLayout &L, int mode, int n_e, int *K,
                      int *g0) {
     *K = 0, *g0 = 0;
@@ -1424,7 +1428,7 @@ static int pair_plan(const PhaseLayout &L, int mode, int n_e, int *K,
     int dev = 0, cus = 256;
     DYF_HIP(hipGetDevice(&dev));
     DYF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    if ((long long)L.nrows * ((n_e + 1) / 2) < 16LL * cus) return DYF_OK;
+    if ((long long)L.nrows * ((n_e + 1) / 2) < 6LL * cus) return DYF_OK;
     int k = 0, g = 0;
     pair_window(L, &k, &g);
     if (k >= 1 && 8 * k >= L.nsteps) *K = k, *g0 = g;
