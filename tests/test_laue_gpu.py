@@ -355,13 +355,14 @@ def test_beam_fan_agrees_across_paths(dev, diamond400, rng, beam):
 
 
 @pytest.mark.parametrize("prec", [FP64, MIXED])
-@pytest.mark.parametrize("n_e", [128, 127])
+@pytest.mark.parametrize("n_e", [128, 127, 38, 36])
 def test_pair_march_equals_single_angle_march(dev, diamond400, rng, prec,
                                               n_e):
-    """A launch with enough (row, angle pair) wavefronts to fill the device
-    twice marches two angles per wavefront while the beam's fan fits half
-    a wavefront (81 rows x 64 pairs here); a one-angle launch never pairs.
-    Every angle of the paired launch, the unpaired last one of an odd count
+    """A launch with at least 6 (row, angle pair) items per CU marches two
+    angles per wavefront while the beam's fan fits half a wavefront (81
+    rows x 64 pairs here; 81 x 19 pairs is just above the rule on a
+    256-CU device, 81 x 18 just below it); a one-angle launch never pairs.
+    Every angle of the launch, the unpaired last one of an odd count
     included, equals its own one-angle solve exactly (np.array_equal: up to
     the sign of zeros outside the fan), and the one-angle march is the one
     checked against the oracle above."""
