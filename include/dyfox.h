@@ -134,9 +134,10 @@ int dyf_laue_phase_from_h(const dyf_laue_args *a, int prec, const void *H_dev,
 /* Fast path: e^{iH} table evaluated from segments with no H grid stored
  * (FP64 Volterra, 2 pi reduction in FP64 for the FP32/mixed table).
  * Entries outside the Borrmann fan of a->sheet_hw's incident beam, where
- * the march's fields are exact zeros, hold 0 instead of e^{iH}: the
- * table is for marches with the same beam (dyf_laue_march skips those
- * entries whatever they hold).                                          */
+ * the march's fields are exact zeros, are not evaluated: those before a
+ * lane's first step in the fan are left unwritten, the lane padding past
+ * Nx holds 0.  The table is for marches with the same beam
+ * (dyf_laue_march never reads the unwritten entries).                   */
 int dyf_laue_phase_from_segments(const dyf_laue_args *a, int prec,
                                  const double *segs, int nseg,
                                  const double *g_vec, double nu,

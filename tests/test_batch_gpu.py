@@ -157,3 +157,28 @@ def test_ensemble_widest_buckets_match_oracle(dev):
                       f"rel_rms={r['rel_rms']:.2e}", flush=True)
                 assert r["rel_rms"] < tol and r["max_abs"] < tol, (C, i)
         solver.close()
+
+
+@pytest.mark.parametrize("prec", [FP64, MIXED], ids=["fp64", "mixed"])
+def test_ensemble_reads_nothing_it_did_not_write(dev, prec):
+    """The ensemble's workspace (tables, coefficients, outputs, queues)
+    filled with 0xFF bytes (NaN in FP64 and FP32) before the run: every
+    instance still equals its own solve_dislocation, so no persistent
+    kernel reads an entry its run did not write (the e^{iH} tables leave
+    the entries before a lane's first step in the fan unwritten, except
+    for the FP32-state march at C = 18, which reads them)."""
+    import ctypes as C
+    from dyfox_amd import _native as N
+    picked = ensemble_subset()
+    inst = [p[2] for p in picked]
+    solver = EnsembleSolver(inst, precision=prec)
+    N.call("dyf_memset", C.c_void_p(solver.work.ptr), 0xFF,
+           C.c_size_t(solver.work.nbytes))
+    N.synchronize()
+    outs = [to_numpy(o) for o in solver.run()]
+    for (C_, i, (xtal, grid, segs, s)), got in zip(picked, outs):
+        want = to_numpy(solve_dislocation(segs, s, grid, xtal,
+                                          precision=prec))
+        assert np.isfinite(got).all(), (C_, i)
+        assert np.abs(got - want).max() <= 1e-13 * np.abs(want).max(), (C_, i)
+    solver.close()

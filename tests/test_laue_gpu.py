@@ -379,3 +379,35 @@ def test_pair_march_equals_single_angle_march(dev, diamond400, rng, prec,
         want = oracle_exit(segs, s[[0, n_e - 1]], grid, diamond400)
         _check(full[[0, n_e - 1]], want, 1e-10, f"pair{n_e}")
 
+
+
+@pytest.mark.parametrize("prec", [FP64, MIXED])
+@pytest.mark.parametrize("n_e", [6, 40])
+def test_unwritten_table_entries_are_never_read(dev, diamond400, rng,
+                                                monkeypatch, prec, n_e):
+    """The e^{iH} table leaves the entries before each lane's first step
+    in the beam's fan unwritten.  With the table's buffer filled with NaN
+    bytes first, the one-angle (6 angles) and the paired (40 angles: 81 x
+    20 pairs) marches give exactly the solve of a zero-filled buffer:
+    no march reads an unwritten entry."""
+    from dyfox_amd.laue import solver as LS
+    grid = LaueGrid(diamond400, 80e-6, 0.25e-6, 20e-6)
+    segs = random_segments(rng, diamond400, grid, 2)
+    s = _s(diamond400, np.linspace(-120.0, 120.0, n_e))
+    orig = LS._table
+
+    def filled(byte):
+        def table(nbytes, reuse):
+            buf = orig(nbytes, False)
+            buf.copy_from(np.full(buf.nbytes, byte, np.uint8))
+            return buf
+        return table
+
+    out = {}
+    for byte in (0x00, 0xFF):              # 0xFF...: NaN in FP64 and FP32
+        monkeypatch.setattr(LS, "_table", filled(byte))
+        out[byte] = to_numpy(solve_dislocation(segs, s, grid, diamond400,
+                                               precision=prec))
+    monkeypatch.setattr(LS, "_table", orig)
+    assert np.isfinite(out[0xFF]).all()
+    assert np.array_equal(out[0xFF], out[0x00])
