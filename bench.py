@@ -61,6 +61,10 @@ def parse():
                     help="run only the rocking-scan solves (FP64 then "
                          "mixed; one warm-up and five timed each), for a "
                          "kernel trace, and exit")
+    ap.add_argument("--probe-worked-example", action="store_true",
+                    help="run only the 200 um worked example's FP64 solves "
+                         "(one warm-up and five timed), for a kernel trace, "
+                         "and exit")
     ap.add_argument("--probe-matvec", action="store_true",
                     help="run only the 20 um Bragg matvec timing (20 "
                          "launches of the full 1312-row march), for "
@@ -562,6 +566,23 @@ def probe_laue():
               f"{p:.2f} ms, march {m:.2f} ms", flush=True)
 
 
+def probe_worked_example():
+    """The 200 um worked example's FP64 solves alone (one warm-up and
+    five timed), so a kernel trace's averages are this workload's."""
+    from dyfox_amd import _native as N
+    from dyfox_amd.backend import FP64, device
+    from dyfox_amd.crystal import DIAMOND, CrystalParams
+    from dyfox_amd.workloads import laue_worked_example
+    device()
+    xtal = CrystalParams.from_reflection(DIAMOND, (4, 0, 0), 17.0,
+                                         f_prime=0.003, f_dprime=0.002)
+    g2, seg2, s2 = laue_worked_example(xtal)
+    w, p, m, out = time_laue(N, seg2, s2, g2, xtal, FP64, 5, 1)
+    del out
+    print(f"probe-worked-example fp64: step {w * 1e3:.2f} ms, phase "
+          f"{p:.2f} ms, march {m:.2f} ms", flush=True)
+
+
 def probe_gemm(batch=32, repeats=10):
     """Every 3x3 layer's three products as the network issues them (the
     operand layouts and the weight gradient's pixel split of
@@ -636,6 +657,8 @@ def main():
         return probe_bragg()
     if args.probe_laue:
         return probe_laue()
+    if args.probe_worked_example:
+        return probe_worked_example()
     if args.probe_gemm:
         return probe_gemm()
     if args.probe_ensemble:
